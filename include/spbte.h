@@ -1,0 +1,122 @@
+/* spbte.h -- batched, device-resident interface of the MI355X collision operator.
+ *
+ * No counterpart in the reference, which evaluates one cell per call on the CPU
+ * (src/collisions.c:178-221) and spreads cells over MPI ranks
+ * (exec/boltz.c:285-345, src/mesh_setup.c:46-53).  This interface is the
+ * single-GPU batched driver: one call evaluates the operator for a whole batch
+ * of cells (a 1D-3V mesh or a 0D ensemble) that share a velocity grid and a
+ * weight table, so every row of the table streamed from HBM serves many cells.
+ *
+ * Conventions: arrays named d_* are device pointers (from spbte_malloc or any
+ * HIP allocation in this process); a batch is ncell contiguous blocks of N^3
+ * doubles in the reference's (i,j,k) -> k + N(j + N i) order.  Functions return
+ * SPBTE_OK (0) or a negative SPBTE_ERR_* code; spbte_last_error() describes the
+ * last failure of the calling thread.  All work runs on the operator's stream;
+ * batched calls return once the work is queued unless stated otherwise.
+ */
+#ifndef SPBTE_H
+#define SPBTE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPBTE_OK = 0,
+  SPBTE_ERR_ARG = -1,       /* bad argument */
+  SPBTE_ERR_HIP = -2,       /* HIP runtime or launch failure */
+  SPBTE_ERR_NOWEIGHTS = -3, /* no weight table set */
+  SPBTE_ERR_IO = -4,        /* weight file missing or short */
+  SPBTE_ERR_SINGULAR = -5,  /* C C^T singular */
+  SPBTE_ERR_NODEV = -6,     /* no GPU visible */
+  SPBTE_ERR_UNSUPPORTED = -7
+};
+
+/* Precision forms of the weighted sum.  FP64 is the reference method.  FP32
+ * streams W as float and sums in compensated FP32.  FP16X2 streams W as two
+ * FP16 halves (hi + lo) with a power-of-two scale per row.  Transforms,
+ * moments and projection are FP64 in every form. */
+enum { SPBTE_FORM_FP64 = 0, SPBTE_FORM_FP32 = 1, SPBTE_FORM_FP16X2 = 2 };
+
+/* flags of spbte_compute_q */
+enum {
+  SPBTE_MAXPRESERVE = 1, /* ComputeQ_maxPreserve semantics (fused single pass) */
+  SPBTE_CONSERVE = 2,    /* apply the single-species projection to each cell */
+  SPBTE_SELF = 4         /* g == f: d_g is ignored */
+};
+
+typedef struct spbte_op spbte_op;
+
+const char *spbte_last_error(void);
+
+/* Device memory and synchronisation helpers (callers need no HIP headers). */
+int spbte_device_count(int *n);
+int spbte_malloc(void **d_ptr, size_t bytes);
+int spbte_free(void *d_ptr);
+int spbte_memcpy_htod(void *d_dst, const void *src, size_t bytes);
+int spbte_memcpy_dtoh(void *dst, const void *d_src, size_t bytes);
+int spbte_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
+int spbte_op_synchronize(spbte_op *op);
+
+/* One velocity grid: N nodes per axis on [-L_v, L_v] with the caller's v and
+ * eta arrays (either grid convention of src/initializer.c:66-82 or :256-266). */
+int spbte_op_create(spbte_op **op, int N, double L_v, const double *v, const double *eta);
+void spbte_op_destroy(spbte_op *op);
+
+/* Species: Ns masses (Ns = 1 for the batched projection).  Builds C C^T and
+ * its LU factors (src/conserve.c:268-317); masses[0] also sets the mass of the
+ * maxPreserve Maxwellian.  Default: one species of mass 1. */
+int spbte_op_set_species(spbte_op *op, int Ns, const double *masses);
+
+/* Weight table.  set_weights takes N^3*N^3 contiguous host doubles (the file
+ * layout); set_weights_rows takes the reference's N^3 row pointers;
+ * load_weights_file reads a Weights/ file; generate_weights_iso builds the
+ * isotropic table on the GPU (src/weights.c:265-281 closed forms for lambda 0
+ * and 1, quadrature otherwise) for species diameters d_i, d_j and masses m_i, m_j.
+ * get_weights copies the FP64 table back to host memory (contiguous). */
+int spbte_op_set_weights(spbte_op *op, const double *W);
+int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows);
+int spbte_op_load_weights_file(spbte_op *op, const char *path);
+int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, double d_j, double m_i, double m_j);
+int spbte_op_get_weights(spbte_op *op, double *W);
+
+/* Q for ncell cells: d_Q[b] = Q(d_f[b], d_g[b]) (ComputeQ, or maxPreserve with
+ * SPBTE_MAXPRESERVE), optionally projected (SPBTE_CONSERVE). */
+int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags);
+
+/* Single-species projection of ncell cells in place (src/conserve.c:207-264). */
+int spbte_conserve(spbte_op *op, double *d_Q, int ncell);
+
+/* General projection of one group of Ns*Ns arrays (array n*Ns+m), in place. */
+int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups);
+
+/* The transform of src/collisions.c:232-283 on ncell complex cubes
+ * (interleaved re/im doubles); invert 0 forward, 1 inverse.  d_in may equal d_out. */
+int spbte_transform(spbte_op *op, const double *d_in, double *d_out, int ncell, int invert);
+
+/* The weighted sum alone, on transformed inputs (interleaved complex FP64):
+ * d_Qhat[b][r] = sum_c W[r][c] d_Ghat[b][c] d_Fhat[b][s(r,c)]. */
+int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double *d_Qhat, int ncell, int form);
+
+/* Tiles of the weighted-sum kernel: tr output rows per lane, lines z-lines of
+ * output modes per workgroup, tb cells per workgroup.  set_tile pins a tile for
+ * (form, maxpreserve); tr = 0 restores automatic choice.  autotune times every
+ * instantiated tile on the given batch shape and pins the fastest; the chosen
+ * tile is returned in *tr, *lines, *tb. */
+int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int tr, int lines, int tb);
+int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb);
+int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb);
+
+/* Timing on the operator's stream with HIP events: runs spbte_compute_q iters
+ * times and returns the mean wall time per call and the mean duration of the
+ * weighted-sum kernel alone (both in milliseconds). */
+int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
+                         int iters, double *ms_per_call, double *ms_wsum);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

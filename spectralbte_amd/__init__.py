@@ -1,0 +1,406 @@
+"""spectralbte_amd -- MI355X collision operator of SpectralBTE, Python host side.
+
+The product is ``libspectralbte_amd.so`` (C host code + HIP kernels for gfx950),
+whose C ABI is declared in ``include/``.  This module loads it with ctypes and
+offers two views of it:
+
+* the reference's own interface (``initialize_coll``, ``ComputeQ``,
+  ``ComputeQ_maxPreserve``, ``fft3D``, ``initialize_conservation``,
+  ``conserveAllMoments``, ``initialize_weights`` ...), with numpy arrays in place
+  of the C pointers -- the names, argument meanings and error behaviour of
+  src/collisions.h, src/conserve.h and src/weights.h;
+* ``Operator``: the batched, device-resident interface of ``include/spbte.h``.
+
+There is no CPU fallback: if the library is missing, ``lib()`` raises; if no GPU
+is visible, operator creation raises ``SpbteError``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspectralbte_amd.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+FORMS = {"fp64": 0, "fp32": 1, "fp16x2": 2}
+MAXPRESERVE, CONSERVE, SELF = 1, 2, 4
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class SpbteError(RuntimeError):
+    pass
+
+
+def build(force=False):
+    """Compile the library for gfx950 with the Makefile beside this file."""
+    args = ["make", "-s", "-C", _HERE, "-j8"]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True)
+
+
+def lib():
+    """The loaded product library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SpbteError(f"{LIB_PATH} is missing: run spectralbte_amd.build() (the operator has no CPU path)")
+        L = ctypes.CDLL(LIB_PATH)
+        I, D, S = ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+        ip = ctypes.POINTER(ctypes.c_int)
+        sig = {
+            # reference interface
+            "initialize_coll": (None, [I, D, _dp, _dp]),
+            "dealloc_coll": (None, []),
+            "ComputeQ": (None, [_dp, _dp, _dp, ctypes.POINTER(_dp)]),
+            "ComputeQ_maxPreserve": (None, [_dp, _dp, _dp, ctypes.POINTER(_dp)]),
+            "fft3D": (None, [_dp, _dp, I]),
+            "ifft3D": (None, [_dp, _dp]),
+            "initialize_conservation": (None, [I, D, _dp, _vp, I]),
+            "initialize_conservation_fast": (None, [I, D, _dp]),
+            "dealloc_conservation": (None, []),
+            "factorMatrixIntoLU": (None, [I, D]),
+            "solveWithCCt": (None, [I, _dp]),
+            "conserveAllMoments": (None, [ctypes.POINTER(_dp)]),
+            "createCCtAndPivot": (None, []),
+            "alloc_weights": (None, [I, _vp, I]),
+            "generate_conv_weights_iso": (None, [ctypes.POINTER(_dp)]),
+            "dealloc_weights": (None, [I, ctypes.POINTER(_dp)]),
+            "ghat": (D, [D, _vp]),
+            "gHat3": (D, [D, D, D, D, D, D]),
+            # batched interface
+            "spbte_last_error": (ctypes.c_char_p, []),
+            "spbte_device_count": (I, [ip]),
+            "spbte_malloc": (I, [ctypes.POINTER(_vp), S]),
+            "spbte_free": (I, [_vp]),
+            "spbte_memcpy_htod": (I, [_vp, _vp, S]),
+            "spbte_memcpy_dtoh": (I, [_vp, _vp, S]),
+            "spbte_memcpy_dtod": (I, [_vp, _vp, S]),
+            "spbte_op_synchronize": (I, [_vp]),
+            "spbte_op_create": (I, [ctypes.POINTER(_vp), I, D, _dp, _dp]),
+            "spbte_op_destroy": (None, [_vp]),
+            "spbte_op_set_species": (I, [_vp, I, _dp]),
+            "spbte_op_set_weights": (I, [_vp, _dp]),
+            "spbte_op_set_weights_rows": (I, [_vp, ctypes.POINTER(_dp)]),
+            "spbte_op_load_weights_file": (I, [_vp, ctypes.c_char_p]),
+            "spbte_op_generate_weights_iso": (I, [_vp, D, D, D, D, D]),
+            "spbte_op_get_weights": (I, [_vp, _dp]),
+            "spbte_compute_q": (I, [_vp, _vp, _vp, _vp, I, I, I]),
+            "spbte_conserve": (I, [_vp, _vp, I]),
+            "spbte_conserve_species": (I, [_vp, _vp, I]),
+            "spbte_transform": (I, [_vp, _vp, _vp, I, I]),
+            "spbte_qhat": (I, [_vp, _vp, _vp, _vp, I, I]),
+            "spbte_set_tile": (I, [_vp, I, I, I, I, I]),
+            "spbte_autotune": (I, [_vp, I, I, I, ip, ip, ip]),
+            "spbte_get_tile": (I, [_vp, I, I, I, ip, ip, ip]),
+            "spbte_time_compute_q": (I, [_vp, _vp, _vp, _vp, I, I, I, I, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SpbteError(f"spbte error {rc}: {lib().spbte_last_error().decode(errors='replace')}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = lib().spbte_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+# ------------------------------------------------------------ device arrays
+class DeviceArray:
+    """A float64 device buffer owned by Python (spbte_malloc / spbte_free)."""
+
+    def __init__(self, shape, data=None):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.ptr = _vp()
+        _check(lib().spbte_malloc(ctypes.byref(self.ptr), max(self.nbytes, 8)))
+        if data is not None:
+            self.upload(data)
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = _c(arr)
+        return cls(arr.shape, arr)
+
+    def upload(self, arr):
+        arr = _c(arr)
+        assert arr.nbytes == self.nbytes, (arr.shape, self.shape)
+        _check(lib().spbte_memcpy_htod(self.ptr, arr.ctypes.data_as(_vp), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(lib().spbte_memcpy_dtoh(out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def offset(self, elems):
+        return _vp(self.ptr.value + 8 * int(elems))
+
+    def free(self):
+        if self.ptr:
+            lib().spbte_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------- batched operator
+class Operator:
+    """spbte_op: one velocity grid, its weight table and the batched kernels."""
+
+    def __init__(self, N, L_v, v, eta):
+        self.N = int(N)
+        self.n3 = self.N ** 3
+        self.v = _c(v)
+        self.eta = _c(eta)
+        self.h = _vp()
+        _check(lib().spbte_op_create(ctypes.byref(self.h), self.N, float(L_v), _p(self.v), _p(self.eta)))
+
+    def close(self):
+        if self.h:
+            lib().spbte_op_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_species(self, masses):
+        m = _c(masses)
+        _check(lib().spbte_op_set_species(self.h, len(m), _p(m)))
+
+    def set_weights(self, W):
+        W = _c(W)
+        assert W.size == self.n3 * self.n3
+        _check(lib().spbte_op_set_weights(self.h, _p(W)))
+
+    def load_weights_file(self, path):
+        _check(lib().spbte_op_load_weights_file(self.h, str(path).encode()))
+
+    def generate_weights_iso(self, lam, d_i=2.0, d_j=2.0, m_i=1.0, m_j=1.0):
+        _check(lib().spbte_op_generate_weights_iso(self.h, lam, d_i, d_j, m_i, m_j))
+
+    def get_weights(self):
+        W = np.empty((self.n3, self.n3))
+        _check(lib().spbte_op_get_weights(self.h, _p(W)))
+        return W
+
+    def sync(self):
+        _check(lib().spbte_op_synchronize(self.h))
+
+    # device-pointer level
+    def compute_q_dev(self, d_f, d_g, d_Q, ncell, form="fp64", flags=0):
+        _check(lib().spbte_compute_q(self.h, d_f, d_g, d_Q, int(ncell), FORMS[form], int(flags)))
+
+    # numpy level
+    def compute_q(self, f, g=None, form="fp64", maxpreserve=False, conserve=False):
+        """f, g: (ncell, N^3) or (N,N,N).  Returns Q with f's shape."""
+        shape = np.shape(f)
+        f = _c(f).reshape(-1, self.n3)
+        ncell = f.shape[0]
+        df = DeviceArray.from_host(f)
+        dg = DeviceArray.from_host(_c(g).reshape(-1, self.n3)) if g is not None else None
+        dQ = DeviceArray((ncell, self.n3))
+        flags = (MAXPRESERVE if maxpreserve else 0) | (CONSERVE if conserve else 0) | (SELF if g is None else 0)
+        self.compute_q_dev(df.ptr, dg.ptr if dg is not None else None, dQ.ptr, ncell, form, flags)
+        self.sync()
+        return dQ.download().reshape(shape)
+
+    def conserve(self, Q):
+        shape = np.shape(Q)
+        Q = _c(Q).reshape(-1, self.n3)
+        dQ = DeviceArray.from_host(Q)
+        _check(lib().spbte_conserve(self.h, dQ.ptr, Q.shape[0]))
+        self.sync()
+        return dQ.download().reshape(shape)
+
+    def conserve_species(self, Q, Ns):
+        """Q: (ngroups, Ns*Ns, N^3)."""
+        shape = np.shape(Q)
+        Q = _c(Q).reshape(-1, Ns * Ns * self.n3)
+        dQ = DeviceArray.from_host(Q)
+        _check(lib().spbte_conserve_species(self.h, dQ.ptr, Q.shape[0]))
+        self.sync()
+        return dQ.download().reshape(shape)
+
+    def transform(self, x, invert=False):
+        """x: complex (ncell, N^3) or (N,N,N).  Returns complex with x's shape."""
+        shape = np.shape(x)
+        x = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1, self.n3)
+        d = DeviceArray.from_host(x.view(np.float64))
+        _check(lib().spbte_transform(self.h, d.ptr, d.ptr, x.shape[0], int(bool(invert))))
+        self.sync()
+        return d.download().view(np.complex128).reshape(shape)
+
+    def qhat(self, Fhat, Ghat, form="fp64"):
+        shape = np.shape(Fhat)
+        F = np.ascontiguousarray(Fhat, dtype=np.complex128).reshape(-1, self.n3)
+        G = np.ascontiguousarray(Ghat, dtype=np.complex128).reshape(-1, self.n3)
+        dF = DeviceArray.from_host(F.view(np.float64))
+        dG = DeviceArray.from_host(G.view(np.float64))
+        dQ = DeviceArray(dF.shape)
+        _check(lib().spbte_qhat(self.h, dF.ptr, dG.ptr, dQ.ptr, F.shape[0], FORMS[form]))
+        self.sync()
+        return dQ.download().view(np.complex128).reshape(shape)
+
+    def set_tile(self, form, maxpreserve, tr, lines, tb):
+        _check(lib().spbte_set_tile(self.h, FORMS[form], int(maxpreserve), tr, lines, tb))
+
+    def get_tile(self, ncell, form, maxpreserve=False):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().spbte_get_tile(self.h, ncell, FORMS[form], int(maxpreserve), ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, ncell, form, maxpreserve=False):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().spbte_autotune(self.h, ncell, FORMS[form], int(maxpreserve), ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def time_compute_q(self, d_f, d_g, d_Q, ncell, form="fp64", flags=0, iters=3):
+        tot, ker = ctypes.c_double(), ctypes.c_double()
+        _check(lib().spbte_time_compute_q(self.h, d_f, d_g, d_Q, int(ncell), FORMS[form], int(flags), int(iters),
+                                          ctypes.byref(tot), ctypes.byref(ker)))
+        return tot.value, ker.value
+
+
+# ------------------------------------------------ reference interface (numpy)
+class Species(ctypes.Structure):
+    """Binary image of the reference's species struct (src/species.h:11-26)."""
+    _fields_ = [("id", ctypes.c_size_t), ("num_levels", ctypes.c_size_t), ("lev_id", ctypes.c_void_p),
+                ("Rgas", ctypes.c_double), ("mass", ctypes.c_double), ("mm", ctypes.c_double),
+                ("d_ref", ctypes.c_double), ("T_ref", ctypes.c_double), ("mu_ref", ctypes.c_double),
+                ("omega", ctypes.c_double), ("E0", ctypes.c_double), ("Ei", ctypes.c_void_p),
+                ("gi", ctypes.c_void_p), ("name", ctypes.c_char * 80)]
+
+
+def default_species(i=0):
+    """src/species.c:28-43 defaults."""
+    s = Species()
+    s.id = i
+    s.Rgas = 1.0
+    s.mass = 1.0
+    s.mm = 1.0
+    s.d_ref = 2.0
+    s.T_ref = 1.0
+    s.mu_ref = 1.0
+    s.E0 = 0.0
+    s.name = b"default"
+    return s
+
+
+class WeightRows:
+    """A contiguous (N^3, N^3) numpy table seen as the reference's double** rows."""
+
+    def __init__(self, W):
+        self.W = _c(W)
+        n3 = self.W.shape[0]
+        base = self.W.ctypes.data
+        self.rows = (_dp * n3)(*[ctypes.cast(base + 8 * n3 * r, _dp) for r in range(n3)])
+
+    @property
+    def ptr(self):
+        return ctypes.cast(self.rows, ctypes.POINTER(_dp))
+
+
+class _RefState:
+    v = None
+    eta = None
+    cons_v = None
+    species = None
+
+
+_ref = _RefState()
+
+
+def initialize_coll(N, L_v, v, eta):
+    """src/collisions.h:27.  v and eta are kept alive here (the C side borrows them)."""
+    _ref.v = _c(v)
+    _ref.eta = _c(eta)
+    lib().initialize_coll(int(N), float(L_v), _p(_ref.v), _p(_ref.eta))
+
+
+def dealloc_coll():
+    lib().dealloc_coll()
+
+
+def ComputeQ(f, g, W):
+    """src/collisions.h:38 with a WeightRows (or contiguous table)."""
+    rows = W if isinstance(W, WeightRows) else WeightRows(W)
+    f = _c(f)
+    g = f if g is f else _c(g)
+    Q = np.zeros_like(f)
+    lib().ComputeQ(_p(f), _p(g), _p(Q), rows.ptr)
+    return Q
+
+
+def ComputeQ_maxPreserve(f, g, W):
+    rows = W if isinstance(W, WeightRows) else WeightRows(W)
+    f = _c(f)
+    g = f if g is f else _c(g)
+    Q = np.zeros_like(f)
+    lib().ComputeQ_maxPreserve(_p(f), _p(g), _p(Q), rows.ptr)
+    return Q
+
+
+def fft3D(x, invert):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    out = np.empty_like(a)
+    lib().fft3D(_p(a.view(np.float64)), _p(out.view(np.float64)), int(invert))
+    return out
+
+
+def initialize_conservation(N, h_v, v, masses=(1.0,)):
+    _ref.cons_v = _c(v)
+    sp = (Species * len(masses))()
+    for i, m in enumerate(masses):
+        sp[i] = default_species(i)
+        sp[i].mass = m
+    _ref.species = sp
+    lib().initialize_conservation(int(N), float(h_v), _p(_ref.cons_v), ctypes.cast(sp, _vp), len(masses))
+
+
+def dealloc_conservation():
+    lib().dealloc_conservation()
+
+
+def conserveAllMoments(Qs):
+    """Qs: list of Ns*Ns arrays (projected copies are returned)."""
+    arrs = [_c(q).copy() for q in Qs]
+    ptrs = (_dp * len(arrs))(*[_p(a) for a in arrs])
+    lib().conserveAllMoments(ctypes.cast(ptrs, ctypes.POINTER(_dp)))
+    return arrs
+
+
+def solveWithCCt(b):
+    b = _c(b).copy()
+    lib().solveWithCCt(len(b), _p(b))
+    return b

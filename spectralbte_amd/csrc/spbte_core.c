@@ -1,0 +1,728 @@
+/* spbte_core.c -- host side of the batched MI355X collision operator (C99).
+ *
+ * Owns the per-grid device state (transform matrices, weight tables in each
+ * precision form, projection factors, scratch), sequences the kernels of one
+ * operator evaluation on the operator's HIP stream, picks and tunes the tile
+ * of the weighted-sum kernel, and implements the public interface of spbte.h.
+ * The reference-compatible entry points (collisions.c, conserve.c, weights.c)
+ * are thin wrappers over this file.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "spbte.h"
+#include "spbte_internal.h"
+
+/* ------------------------------------------------------------------ errors */
+static __thread char g_err[512];
+
+const char *spbte_last_error(void) { return g_err; }
+
+static int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return fail(SPBTE_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                            \
+  } while (0)
+
+#define LCHK(expr)                                                                                           \
+  do {                                                                                                       \
+    int e_ = (expr);                                                                                         \
+    if (e_ != 0) return fail(SPBTE_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString((hipError_t)e_),      \
+                             __FILE__, __LINE__);                                                            \
+  } while (0)
+
+/* --------------------------------------------------------------- op state */
+#define NFORMS 3
+#define MAXJOBS 4
+
+struct spbte_op {
+  spbte_grid_dev g;
+  int N;
+  size_t n3;
+  double *v, *eta; /* host copies */
+  hipStream_t s;
+  /* weight table: FP64 master copy and derived forms */
+  double *dW64;
+  void *dWf[NFORMS]; /* [0] aliases dW64; [1] float; [2] (hi,lo) pairs + row scales */
+  int have_w;
+  /* conservation */
+  int Ns;
+  double masses[SPBTE_MAX_NS];
+  double LU[(SPBTE_MAX_NS + 4) * (SPBTE_MAX_NS + 4)];
+  int pivot[SPBTE_MAX_NS + 4];
+  /* scratch, sized for cap cells */
+  int cap;
+  double *d_mom;     /* 2 * cap * SPBTE_MOM_STRIDE */
+  void *d_xs;        /* transform scratch: MAXJOBS * cap * n3 double2 */
+  void *d_FG;        /* MAXJOBS * cap * n3 double2 (FP32 forms use half) */
+  void *d_Qh;        /* cap * n3 double2 */
+  /* tiles pinned per (form, nprod-1); tr == 0 -> automatic */
+  spbte_tile pin[NFORMS][2];
+  /* autotuned tiles per (form, nprod-1, ncell bucket) */
+  spbte_tile tuned[NFORMS][2][16];
+};
+
+int spbte_device_count(int *n) {
+  HIPCHK(hipGetDeviceCount(n));
+  return SPBTE_OK;
+}
+int spbte_malloc(void **p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes ? bytes : 1));
+  return SPBTE_OK;
+}
+int spbte_free(void *p) {
+  if (p) HIPCHK(hipFree(p));
+  return SPBTE_OK;
+}
+int spbte_memcpy_htod(void *d, const void *s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyHostToDevice));
+  return SPBTE_OK;
+}
+int spbte_memcpy_dtoh(void *d, const void *s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost));
+  return SPBTE_OK;
+}
+int spbte_memcpy_dtod(void *d, const void *s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToDevice));
+  return SPBTE_OK;
+}
+int spbte_op_synchronize(spbte_op *op) {
+  HIPCHK(hipStreamSynchronize(op->s));
+  return SPBTE_OK;
+}
+
+hipStream_t spbte_op_stream(spbte_op *op) { return op->s; }
+const spbte_grid_dev *spbte_op_grid(spbte_op *op) { return &op->g; }
+
+/* ------------------------------------------------------ transform matrices */
+
+/* Per-axis matrix of the transform of src/collisions.c:232-283 (see
+ * k_transform.hip): M[k][j] = exp(i s L_end a_k) exp(-+2 pi i (jk mod N)/N)
+ * exp(i s j L_start delta) w_j, accumulated phase in long double. */
+static void build_matrix(int N, const double *a, double delta, double Lstart, double Lend, double sign, double dsgn,
+                         double *M /* 2*N*N */) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < N; k++)
+    for (int j = 0; j < N; j++) {
+      long double ph = (long double)sign * Lend * a[k] + (long double)dsgn * 2.0L * pi * (long double)((j * k) % N) / N +
+                       (long double)sign * j * Lstart * delta;
+      double w = (j == 0 || j == N - 1) ? 0.5 : 1.0;
+      M[2 * (k * N + j)] = (double)(cosl(ph) * w);
+      M[2 * (k * N + j) + 1] = (double)(sinl(ph) * w);
+    }
+}
+
+/* ---------------------------------------------- LU (src/conserve.c:89-168) */
+
+/* Scaled partial pivoting after Atkinson p.520, as the reference: the pivot
+ * row is the first row below whose scaled entry exceeds the diagonal's. */
+int spbte_host_lu_factor(int n, double *A, int *pivot) {
+  double s[SPBTE_MAX_NS + 4];
+  for (int i = 0; i < n; i++) {
+    s[i] = fabs(A[i * n]);
+    for (int j = 0; j < n; j++)
+      if (s[i] < fabs(A[i * n + j])) s[i] = fabs(A[i * n + j]);
+  }
+  for (int k = 0; k < n - 1; k++) {
+    double ck = fabs(A[k * n + k] / s[k]);
+    int i0 = k, found = 0;
+    for (int i = k; i < n; i++)
+      if (ck < fabs(A[i * n + k] / s[i])) {
+        ck = fabs(A[i * n + k] / s[i]);
+        if (!found) { i0 = i; found = 1; }
+      }
+    pivot[k] = i0;
+    if (ck == 0.0) return -1;
+    if (i0 != k) {
+      for (int j = k; j < n; j++) { double t = A[k * n + j]; A[k * n + j] = A[i0 * n + j]; A[i0 * n + j] = t; }
+      double t = s[k]; s[k] = s[i0]; s[i0] = t;
+    }
+    for (int i = k + 1; i < n; i++) {
+      double m = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = m;
+      for (int j = k + 1; j < n; j++) A[i * n + j] -= m * A[k * n + j];
+    }
+  }
+  pivot[n - 1] = n - 1;
+  return 0;
+}
+
+/* src/conserve.c:173-202 */
+void spbte_host_lu_solve(int n, const double *LU, const int *pivot, double *b) {
+  for (int k = 0; k < n - 1; k++) {
+    int i = pivot[k];
+    if (i != k) { double t = b[i]; b[i] = b[k]; b[k] = t; }
+    for (i = k + 1; i < n; i++) b[i] -= LU[i * n + k] * b[k];
+  }
+  b[n - 1] = b[n - 1] / LU[(n - 1) * n + n - 1];
+  for (int i = n - 2; i >= 0; i--) {
+    double sum = 0.0;
+    for (int j = i + 1; j < n; j++) sum += LU[i * n + j] * b[j];
+    b[i] = 1.0 / LU[i * n + i] * (b[i] - sum);
+  }
+}
+
+/* C C^T of src/conserve.c:268-305 (unfactored), n = Ns + 4. */
+void spbte_host_cct(int N, const double *v, int Ns, const double *masses, double *CCt) {
+  int n = Ns + 4;
+  double dv = v[1] - v[0];
+  for (int a = 0; a < n * n; a++) CCt[a] = 0.0;
+  for (int m = 0; m < Ns; m++)
+    for (int i = 0; i < N; i++)
+      for (int j = 0; j < N; j++)
+        for (int k = 0; k < N; k++) {
+          double wi = (i == 0 || i == N - 1) ? 0.5 : 1.0, wj = (j == 0 || j == N - 1) ? 0.5 : 1.0,
+                 wk = (k == 0 || k == N - 1) ? 0.5 : 1.0;
+          double p = wi * wj * wk * dv * dv * dv * masses[m];
+          double c[SPBTE_MAX_NS + 4];
+          for (int q = 0; q < n; q++) c[q] = 0.0;
+          c[m] = p;
+          c[Ns] = p * v[i];
+          c[Ns + 1] = p * v[j];
+          c[Ns + 2] = p * v[k];
+          c[Ns + 3] = p * 0.5 * (v[i] * v[i] + v[j] * v[j] + v[k] * v[k]);
+          for (int a = 0; a < n; a++)
+            for (int b = 0; b < n; b++) CCt[a * n + b] += c[a] * c[b];
+        }
+}
+
+int spbte_op_set_species(spbte_op *op, int Ns, const double *masses) {
+  if (Ns < 1 || Ns > SPBTE_MAX_NS) return fail(SPBTE_ERR_ARG, "Ns=%d out of range [1,%d]", Ns, SPBTE_MAX_NS);
+  op->Ns = Ns;
+  for (int m = 0; m < Ns; m++) op->masses[m] = masses ? masses[m] : 1.0;
+  spbte_host_cct(op->N, op->v, Ns, op->masses, op->LU);
+  if (spbte_host_lu_factor(Ns + 4, op->LU, op->pivot) != 0) return fail(SPBTE_ERR_SINGULAR, "C C^T is singular");
+  op->g.mass = op->masses[0];
+  op->g.kb = (op->masses[0] == 1.0) ? 1.0 : 1.380658e-23;
+  return SPBTE_OK;
+}
+
+/* ----------------------------------------------------------- create/destroy */
+int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const double *eta) {
+  *out = NULL;
+  if (N < 2 || N > SPBTE_MAX_N || !v || !eta) return fail(SPBTE_ERR_ARG, "bad grid (N=%d)", N);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SPBTE_ERR_NODEV, "no GPU visible to HIP");
+  spbte_op *op = calloc(1, sizeof *op);
+  op->N = N;
+  op->n3 = (size_t)N * N * N;
+  op->v = malloc(sizeof(double) * N);
+  op->eta = malloc(sizeof(double) * N);
+  memcpy(op->v, v, sizeof(double) * N);
+  memcpy(op->eta, eta, sizeof(double) * N);
+  spbte_grid_dev *g = &op->g;
+  g->N = N;
+  g->Lv = L_v;
+  g->hv = v[1] - v[0];
+  g->heta = eta[1] - eta[0];
+  const double scale3 = pow(1.0 / sqrt(2.0 * M_PI), 3.0); /* src/collisions.c:46 */
+  g->pre_fwd = scale3 * g->hv * g->hv * g->hv;
+  g->pre_inv = scale3 * g->heta * g->heta * g->heta;
+  double *Mf = malloc(sizeof(double) * 2 * N * N), *Mi = malloc(sizeof(double) * 2 * N * N), w[SPBTE_MAX_N];
+  const double Leta = -eta[0];
+  build_matrix(N, eta, g->hv, Leta, L_v, 1.0, -1.0, Mf);
+  build_matrix(N, v, g->heta, L_v, Leta, -1.0, 1.0, Mi);
+  for (int i = 0; i < N; i++) w[i] = (i == 0 || i == N - 1) ? 0.5 : 1.0;
+  int rc = SPBTE_OK;
+  if (hipStreamCreate(&op->s) != hipSuccess || hipMalloc(&g->d_mat_fwd, sizeof(double) * 2 * N * N) != hipSuccess ||
+      hipMalloc(&g->d_mat_inv, sizeof(double) * 2 * N * N) != hipSuccess ||
+      hipMalloc(&g->d_v, sizeof(double) * N) != hipSuccess || hipMalloc(&g->d_w, sizeof(double) * N) != hipSuccess ||
+      hipMemcpy(g->d_mat_fwd, Mf, sizeof(double) * 2 * N * N, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->d_mat_inv, Mi, sizeof(double) * 2 * N * N, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->d_v, v, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->d_w, w, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(SPBTE_ERR_HIP, "device setup failed: %s", hipGetErrorString(hipGetLastError()));
+  free(Mf);
+  free(Mi);
+  if (rc != SPBTE_OK) {
+    spbte_op_destroy(op);
+    return rc;
+  }
+  double one = 1.0;
+  rc = spbte_op_set_species(op, 1, &one);
+  if (rc != SPBTE_OK) {
+    spbte_op_destroy(op);
+    return rc;
+  }
+  *out = op;
+  return SPBTE_OK;
+}
+
+static void free_weights(spbte_op *op) {
+  if (op->dW64) hipFree(op->dW64);
+  for (int f = 1; f < NFORMS; f++)
+    if (op->dWf[f]) hipFree(op->dWf[f]);
+  op->dW64 = NULL;
+  for (int f = 0; f < NFORMS; f++) op->dWf[f] = NULL;
+  op->have_w = 0;
+}
+
+static void free_scratch(spbte_op *op) {
+  if (op->d_mom) hipFree(op->d_mom);
+  if (op->d_xs) hipFree(op->d_xs);
+  if (op->d_FG) hipFree(op->d_FG);
+  if (op->d_Qh) hipFree(op->d_Qh);
+  op->d_mom = NULL;
+  op->d_xs = op->d_FG = op->d_Qh = NULL;
+  op->cap = 0;
+}
+
+void spbte_op_destroy(spbte_op *op) {
+  if (!op) return;
+  if (op->s) hipStreamSynchronize(op->s);
+  free_weights(op);
+  free_scratch(op);
+  if (op->g.d_mat_fwd) hipFree(op->g.d_mat_fwd);
+  if (op->g.d_mat_inv) hipFree(op->g.d_mat_inv);
+  if (op->g.d_v) hipFree(op->g.d_v);
+  if (op->g.d_w) hipFree(op->g.d_w);
+  if (op->s) hipStreamDestroy(op->s);
+  free(op->v);
+  free(op->eta);
+  free(op);
+}
+
+/* Largest batch evaluated in one pass; bigger batches are processed in chunks. */
+static int chunk_cells(const spbte_op *op) {
+  size_t per_cell = op->n3 * 16 * (2 * MAXJOBS + 1);
+  size_t budget = (size_t)6 << 30; /* 6 GiB of scratch */
+  size_t c = budget / per_cell;
+  if (c > 4096) c = 4096;
+  if (c < 1) c = 1;
+  return (int)c;
+}
+
+static int ensure_scratch(spbte_op *op, int ncell) {
+  if (ncell <= op->cap) return SPBTE_OK;
+  HIPCHK(hipStreamSynchronize(op->s));
+  free_scratch(op);
+  size_t n = (size_t)ncell * op->n3;
+  HIPCHK(hipMalloc((void **)&op->d_mom, sizeof(double) * 2 * ncell * SPBTE_MOM_STRIDE));
+  HIPCHK(hipMalloc(&op->d_xs, 16 * n * MAXJOBS));
+  HIPCHK(hipMalloc(&op->d_FG, 16 * n * MAXJOBS));
+  HIPCHK(hipMalloc(&op->d_Qh, 16 * n));
+  op->cap = ncell;
+  return SPBTE_OK;
+}
+
+/* ------------------------------------------------------------------ weights */
+static int alloc_w64(spbte_op *op) {
+  if (op->dW64) {
+    for (int f = 1; f < NFORMS; f++)
+      if (op->dWf[f]) { hipFree(op->dWf[f]); op->dWf[f] = NULL; }
+    return SPBTE_OK;
+  }
+  HIPCHK(hipMalloc((void **)&op->dW64, sizeof(double) * op->n3 * op->n3));
+  op->dWf[0] = op->dW64;
+  return SPBTE_OK;
+}
+
+int spbte_op_set_weights(spbte_op *op, const double *W) {
+  int rc = alloc_w64(op);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(op->dW64, W, sizeof(double) * op->n3 * op->n3, hipMemcpyHostToDevice));
+  op->have_w = 1;
+  return SPBTE_OK;
+}
+
+int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows) {
+  int rc = alloc_w64(op);
+  if (rc) return rc;
+  const size_t n3 = op->n3, stage_rows = (((size_t)64 << 20) / (n3 * 8)) ? ((size_t)64 << 20) / (n3 * 8) : 1;
+  double *stage = NULL;
+  HIPCHK(hipHostMalloc((void **)&stage, sizeof(double) * n3 * stage_rows, hipHostMallocDefault));
+  for (size_t r0 = 0; r0 < n3; r0 += stage_rows) {
+    size_t nr = (r0 + stage_rows <= n3) ? stage_rows : n3 - r0;
+    for (size_t r = 0; r < nr; r++) memcpy(stage + r * n3, rows[r0 + r], sizeof(double) * n3);
+    hipError_t e = hipMemcpy(op->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipHostFree(stage);
+      return fail(SPBTE_ERR_HIP, "weight upload: %s", hipGetErrorString(e));
+    }
+  }
+  hipHostFree(stage);
+  op->have_w = 1;
+  return SPBTE_OK;
+}
+
+int spbte_op_load_weights_file(spbte_op *op, const char *path) {
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return fail(SPBTE_ERR_IO, "cannot open weight file %s", path);
+  int rc = alloc_w64(op);
+  if (rc) { fclose(fp); return rc; }
+  const size_t n3 = op->n3, stage_rows = (((size_t)64 << 20) / (n3 * 8)) ? ((size_t)64 << 20) / (n3 * 8) : 1;
+  double *stage = NULL;
+  if (hipHostMalloc((void **)&stage, sizeof(double) * n3 * stage_rows, hipHostMallocDefault) != hipSuccess) {
+    fclose(fp);
+    return fail(SPBTE_ERR_HIP, "pinned staging allocation failed");
+  }
+  for (size_t r0 = 0; r0 < n3; r0 += stage_rows) {
+    size_t nr = (r0 + stage_rows <= n3) ? stage_rows : n3 - r0;
+    if (fread(stage, sizeof(double), n3 * nr, fp) != n3 * nr) {
+      hipHostFree(stage);
+      fclose(fp);
+      return fail(SPBTE_ERR_IO, "weight file %s is short", path);
+    }
+    hipError_t e = hipMemcpy(op->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipHostFree(stage);
+      fclose(fp);
+      return fail(SPBTE_ERR_HIP, "weight upload: %s", hipGetErrorString(e));
+    }
+  }
+  hipHostFree(stage);
+  fclose(fp);
+  op->have_w = 1;
+  return SPBTE_OK;
+}
+
+int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, double d_j, double m_i, double m_j) {
+  int rc = alloc_w64(op);
+  if (rc) return rc;
+  double *d_eta = NULL;
+  HIPCHK(hipMalloc((void **)&d_eta, sizeof(double) * op->N));
+  HIPCHK(hipMemcpy(d_eta, op->eta, sizeof(double) * op->N, hipMemcpyHostToDevice));
+  const double dfac = 0.25 * pow(0.5 * (d_i + d_j), 2); /* src/weights.c:277 */
+  const double mu = m_j / (m_i + m_j);                  /* src/weights.c:190 */
+  int e = spbte_launch_weights_iso(&op->g, d_eta, lambda, dfac, mu, op->dW64, op->s);
+  hipError_t e2 = hipStreamSynchronize(op->s);
+  hipFree(d_eta);
+  if (e != 0 || e2 != hipSuccess)
+    return fail(SPBTE_ERR_HIP, "weight generation: %s", hipGetErrorString(e ? (hipError_t)e : e2));
+  op->have_w = 1;
+  return SPBTE_OK;
+}
+
+int spbte_op_get_weights(spbte_op *op, double *W) {
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  HIPCHK(hipStreamSynchronize(op->s));
+  HIPCHK(hipMemcpy(W, op->dW64, sizeof(double) * op->n3 * op->n3, hipMemcpyDeviceToHost));
+  return SPBTE_OK;
+}
+
+int spbte_op_get_weights_rows(spbte_op *op, double **rows) {
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  HIPCHK(hipStreamSynchronize(op->s));
+  for (size_t r = 0; r < op->n3; r++)
+    HIPCHK(hipMemcpy(rows[r], op->dW64 + r * op->n3, sizeof(double) * op->n3, hipMemcpyDeviceToHost));
+  return SPBTE_OK;
+}
+
+static int weights_form(spbte_op *op, int form, const void **W, const float **rowscale) {
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  *rowscale = NULL;
+  if (form == SPBTE_FORM_FP64) {
+    *W = op->dW64;
+    return SPBTE_OK;
+  }
+  const size_t n = op->n3 * op->n3;
+  if (!op->dWf[form]) {
+    size_t bytes = (form == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
+    HIPCHK(hipMalloc(&op->dWf[form], bytes));
+    LCHK(spbte_launch_convert_weights(op->dW64, op->dWf[form], form, op->n3, op->n3, op->s));
+  }
+  *W = op->dWf[form];
+  if (form == SPBTE_FORM_FP16X2) *rowscale = (const float *)((const unsigned *)op->dWf[form] + n);
+  return SPBTE_OK;
+}
+
+/* ------------------------------------------------------------------ tiles */
+static int bucket(int ncell) {
+  int b = 0;
+  while ((1 << b) < ncell && b < 15) b++;
+  return b;
+}
+
+/* Default tile: enough workgroups to fill 256 CUs, cells per workgroup as the
+ * batch allows (each streamed W value then feeds tb cells). */
+static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
+  static const spbte_tile c32_1[] = {{4, 4, 4}, {4, 2, 4}, {4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
+  static const spbte_tile c32_1_64[] = {{4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
+  static const spbte_tile c32_2[] = {{4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
+  static const spbte_tile c24_1[] = {{6, 8, 4}, {6, 8, 2}, {6, 8, 1}};
+  static const spbte_tile c24_2[] = {{6, 8, 2}, {6, 8, 1}};
+  static const spbte_tile c16_1[] = {{4, 8, 4}, {4, 8, 2}, {4, 4, 2}, {4, 4, 1}};
+  static const spbte_tile c16_2[] = {{4, 8, 2}, {4, 4, 2}, {4, 4, 1}};
+  static const spbte_tile c8[] = {{2, 8, 2}};
+  const spbte_tile *c = NULL;
+  int nc = 0;
+  switch (N) {
+    case 32:
+      if (nprod == 2) { c = c32_2; nc = 4; }
+      else if (form == 0) { c = c32_1_64; nc = 4; }
+      else { c = c32_1; nc = 6; }
+      break;
+    case 24: if (nprod == 2) { c = c24_2; nc = 2; } else { c = c24_1; nc = 3; } break;
+    case 16: if (nprod == 2) { c = c16_2; nc = 3; } else { c = c16_1; nc = 4; } break;
+    case 8: c = c8; nc = 1; break;
+    default: return 0;
+  }
+  for (int i = 0; i < nc; i++) {
+    if (!spbte_wsum_tile_supported(N, form, nprod, &c[i])) continue;
+    int ncg = (ncell + c[i].tb - 1) / c[i].tb;
+    long wgs = (long)N * (N / c[i].lines) * ncg;
+    if (c[i].tb > ncell && i + 1 < nc) continue;
+    if (wgs >= 1024 || i + 1 == nc) { *t = c[i]; return 1; }
+  }
+  for (int i = nc - 1; i >= 0; i--)
+    if (spbte_wsum_tile_supported(N, form, nprod, &c[i])) { *t = c[i]; return 1; }
+  return 0;
+}
+
+static int pick_tile(spbte_op *op, int ncell, int form, int nprod, spbte_tile *t) {
+  if (op->pin[form][nprod - 1].tr) { *t = op->pin[form][nprod - 1]; return 1; }
+  spbte_tile *tt = &op->tuned[form][nprod - 1][bucket(ncell)];
+  if (tt->tr) { *t = *tt; return 1; }
+  return default_tile(op->N, form, nprod, ncell, t);
+}
+
+int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int tr, int lines, int tb) {
+  if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
+  int np = maxpreserve ? 2 : 1;
+  spbte_tile t = {tr, lines, tb};
+  if (tr && !spbte_wsum_tile_supported(op->N, form, np, &t))
+    return fail(SPBTE_ERR_UNSUPPORTED, "tile (%d,%d,%d) not instantiated for N=%d form %d", tr, lines, tb, op->N, form);
+  op->pin[form][np - 1] = t;
+  return SPBTE_OK;
+}
+
+int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb) {
+  spbte_tile t = {0, 0, 0};
+  if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
+  if (!pick_tile(op, ncell, form, maxpreserve ? 2 : 1, &t)) t.tr = t.lines = t.tb = 0;
+  *tr = t.tr; *lines = t.lines; *tb = t.tb;
+  return SPBTE_OK;
+}
+
+/* ---------------------------------------------------------- the operator */
+static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
+                    int ncell) {
+  const void *W;
+  const float *rs;
+  int rc = weights_form(op, form, &W, &rs);
+  if (rc) return rc;
+  spbte_tile t;
+  int have = pick_tile(op, ncell, form, nprod, &t);
+  if (!have && form != SPBTE_FORM_FP64)
+    return fail(SPBTE_ERR_UNSUPPORTED, "no FP32/FP16x2 weighted-sum kernel for N=%d", op->N);
+  spbte_wsum_args a = {W, F0, G0, F1, G1, op->d_Qh, ncell, nprod};
+  LCHK(spbte_launch_wsum_scaled(op->N, form, have ? &t : NULL, &a, rs, op->s));
+  return SPBTE_OK;
+}
+
+/* One chunk of spbte_compute_q.  ev_a/ev_b (optional) bracket the weighted-sum
+ * launch so its duration can be timed on the operator's stream. */
+static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
+                         hipEvent_t ev_a, hipEvent_t ev_b) {
+  const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
+  const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
+  const size_t n = (size_t)ncell * op->n3;
+  const size_t esz = (form == SPBTE_FORM_FP64) ? 16 : 8;
+  char *FG = (char *)op->d_FG;
+  void *B[MAXJOBS];
+  for (int q = 0; q < MAXJOBS; q++) B[q] = FG + (size_t)q * n * esz;
+  spbte_xform_job jobs[MAXJOBS];
+  int nj = 0;
+  const void *F0, *G0, *F1 = NULL, *G1 = NULL;
+  double *mom_f = op->d_mom, *mom_g = op->d_mom + (size_t)ncell * SPBTE_MOM_STRIDE;
+  if (!mp) {
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};
+    F0 = B[0];
+    if (self) G0 = B[0];
+    else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1]}; G0 = B[1]; }
+  } else {
+    /* Q = sum W [ T(g - M_f)[c] T(f)[s] + T(M_g)[c] T(f - M_f)[s] ] */
+    LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, op->s));
+    if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, op->s));
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};        /* F0 = T(f)        */
+    jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1]};       /* F1 = T(f - M_f)  */
+    F0 = B[0];
+    F1 = B[1];
+    if (self) {
+      G0 = B[1];
+      jobs[nj++] = (spbte_xform_job){d_f, mom_f, 2, B[2]};     /* G1 = T(M_f)      */
+      G1 = B[2];
+    } else {
+      jobs[nj++] = (spbte_xform_job){d_g, mom_f, 1, B[2]};     /* G0 = T(g - M_f)  */
+      jobs[nj++] = (spbte_xform_job){d_g, mom_g, 2, B[3]};     /* G1 = T(M_g)      */
+      G0 = B[2];
+      G1 = B[3];
+    }
+  }
+  LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, form != SPBTE_FORM_FP64, op->d_xs, op->s));
+  if (ev_a) HIPCHK(hipEventRecord(ev_a, op->s));
+  int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell);
+  if (ev_b) HIPCHK(hipEventRecord(ev_b, op->s));
+  if (rc) return rc;
+  LCHK(spbte_launch_xform_inv_real(&op->g, op->d_Qh, d_Q, ncell, op->d_xs, op->s));
+  if (flags & SPBTE_CONSERVE) {
+    if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
+    LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
+  }
+  return SPBTE_OK;
+}
+
+int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags) {
+  if (!op || ncell < 0 || form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ncell == 0) return SPBTE_OK;
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  const int ch = chunk_cells(op);
+  int rc = ensure_scratch(op, ncell < ch ? ncell : ch);
+  if (rc) return rc;
+  for (int c0 = 0; c0 < ncell; c0 += ch) {
+    int nc = (ncell - c0 < ch) ? ncell - c0 : ch;
+    const size_t off = (size_t)c0 * op->n3;
+    rc = compute_chunk(op, d_f + off, d_g ? d_g + off : NULL, d_Q + off, nc, form, flags | (d_g ? 0 : SPBTE_SELF), NULL,
+                       NULL);
+    if (rc) return rc;
+  }
+  return SPBTE_OK;
+}
+
+int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
+  if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
+  if (ncell <= 0) return SPBTE_OK;
+  LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
+  return SPBTE_OK;
+}
+
+int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups) {
+  if (ngroups <= 0) return SPBTE_OK;
+  LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, op->s));
+  return SPBTE_OK;
+}
+
+int spbte_transform(spbte_op *op, const double *d_in, double *d_out, int ncell, int invert) {
+  if (ncell <= 0) return SPBTE_OK;
+  const int ch = chunk_cells(op);
+  int rc = ensure_scratch(op, ncell < ch ? ncell : ch);
+  if (rc) return rc;
+  for (int c0 = 0; c0 < ncell; c0 += ch) {
+    int nc = (ncell - c0 < ch) ? ncell - c0 : ch;
+    const size_t off = (size_t)c0 * op->n3 * 2;
+    LCHK(spbte_launch_xform_fwd_complex(&op->g, d_in + off, d_out + off, nc, invert, op->d_xs, op->s));
+  }
+  return SPBTE_OK;
+}
+
+int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double *d_Qhat, int ncell, int form) {
+  if (ncell <= 0) return SPBTE_OK;
+  if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
+  int rc = ensure_scratch(op, ncell);
+  if (rc) return rc;
+  const void *F = d_Fhat, *G = d_Ghat;
+  const size_t n = (size_t)ncell * op->n3;
+  if (form != SPBTE_FORM_FP64) {
+    char *FG = (char *)op->d_FG;
+    LCHK(spbte_launch_c64_to_c32(d_Fhat, FG, n, op->s));
+    LCHK(spbte_launch_c64_to_c32(d_Ghat, FG + 8 * n, n, op->s));
+    F = FG;
+    G = FG + 8 * n;
+  }
+  rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(d_Qhat, op->d_Qh, 16 * n, hipMemcpyDeviceToDevice, op->s));
+  return SPBTE_OK;
+}
+
+/* ------------------------------------------------------------ timing/tuning */
+int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
+                         int iters, double *ms_per_call, double *ms_wsum) {
+  if (iters < 1) return fail(SPBTE_ERR_ARG, "iters < 1");
+  const int ch = chunk_cells(op);
+  if (ncell > ch) return fail(SPBTE_ERR_ARG, "timed batch larger than one chunk (%d cells)", ch);
+  int rc = ensure_scratch(op, ncell);
+  if (rc) return rc;
+  hipEvent_t e0, e1, *ka = calloc(iters, sizeof(hipEvent_t)), *kb = calloc(iters, sizeof(hipEvent_t));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int i = 0; i < iters; i++) {
+    HIPCHK(hipEventCreate(&ka[i]));
+    HIPCHK(hipEventCreate(&kb[i]));
+  }
+  HIPCHK(hipStreamSynchronize(op->s));
+  HIPCHK(hipEventRecord(e0, op->s));
+  for (int i = 0; i < iters && rc == SPBTE_OK; i++)
+    rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ka[i], kb[i]);
+  HIPCHK(hipEventRecord(e1, op->s));
+  HIPCHK(hipEventSynchronize(e1));
+  float tot = 0.f, k = 0.f, ksum = 0.f;
+  HIPCHK(hipEventElapsedTime(&tot, e0, e1));
+  for (int i = 0; i < iters; i++) {
+    if (rc == SPBTE_OK) {
+      HIPCHK(hipEventElapsedTime(&k, ka[i], kb[i]));
+      ksum += k;
+    }
+    hipEventDestroy(ka[i]);
+    hipEventDestroy(kb[i]);
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  free(ka);
+  free(kb);
+  if (rc) return rc;
+  *ms_per_call = tot / iters;
+  *ms_wsum = ksum / iters;
+  return SPBTE_OK;
+}
+
+/* Candidate tiles for the autotuner (must match k_wsum.hip's table). */
+static const spbte_tile k_cand[] = {{4, 1, 1}, {4, 1, 2}, {4, 2, 1}, {4, 2, 2}, {4, 4, 1}, {4, 4, 2}, {4, 2, 4},
+                                    {4, 4, 4}, {8, 4, 2}, {8, 8, 2}, {8, 8, 4}, {8, 4, 4}, {6, 8, 1}, {6, 8, 2},
+                                    {6, 8, 4}, {4, 8, 2}, {4, 8, 4}, {2, 8, 2}, {2, 8, 1}};
+
+int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb) {
+  if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");
+  const int np = maxpreserve ? 2 : 1;
+  int rc = ensure_scratch(op, ncell);
+  if (rc) return rc;
+  const void *W;
+  const float *rs;
+  rc = weights_form(op, form, &W, &rs);
+  if (rc) return rc;
+  /* inputs: whatever the scratch holds; timing does not depend on values */
+  const size_t n = (size_t)ncell * op->n3;
+  const size_t esz = (form == SPBTE_FORM_FP64) ? 16 : 8;
+  char *FG = (char *)op->d_FG;
+  HIPCHK(hipMemsetAsync(FG, 0, esz * n * 4, op->s));
+  const void *F0 = FG, *G0 = FG + esz * n, *F1 = FG + 2 * esz * n, *G1 = FG + 3 * esz * n;
+  spbte_tile best = {0, 0, 0};
+  float best_ms = 1e30f;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  for (size_t i = 0; i < sizeof k_cand / sizeof k_cand[0]; i++) {
+    const spbte_tile *t = &k_cand[i];
+    if (!spbte_wsum_tile_supported(op->N, form, np, t)) continue;
+    if (t->tb > 2 * ncell) continue;
+    spbte_wsum_args args = {W, F0, G0, np == 2 ? F1 : NULL, np == 2 ? G1 : NULL, op->d_Qh, ncell, np};
+    LCHK(spbte_launch_wsum_scaled(op->N, form, t, &args, rs, op->s)); /* warm-up */
+    HIPCHK(hipEventRecord(a, op->s));
+    const int reps = 2;
+    for (int r = 0; r < reps; r++) LCHK(spbte_launch_wsum_scaled(op->N, form, t, &args, rs, op->s));
+    HIPCHK(hipEventRecord(b, op->s));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    ms /= reps;
+    if (ms < best_ms) { best_ms = ms; best = *t; }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (!best.tr) return fail(SPBTE_ERR_UNSUPPORTED, "no tiled kernel for N=%d", op->N);
+  op->tuned[form][np - 1][bucket(ncell)] = best;
+  *tr = best.tr; *lines = best.lines; *tb = best.tb;
+  return SPBTE_OK;
+}

@@ -1,0 +1,90 @@
+/* spbte_internal.h -- declarations shared by the host C layer and the HIP
+ * kernel launchers of libspectralbte_amd.  Plain C types only. */
+#ifndef SPBTE_INTERNAL_H
+#define SPBTE_INTERNAL_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPBTE_MAX_N 64
+#define SPBTE_MAX_NS 8
+
+/* Per-grid device tables for the transform, moments and projection. */
+typedef struct {
+  int N;
+  double Lv, hv, heta;
+  double pre_fwd, pre_inv; /* (2 pi)^-3/2 h^3 of each direction */
+  double mass, kb;         /* Maxwellian of the maxPreserve split */
+  void *d_mat_fwd;         /* double2[N*N]: per-axis forward matrix */
+  void *d_mat_inv;         /* double2[N*N]: per-axis inverse matrix */
+  void *d_v;               /* double[N] */
+  void *d_w;               /* double[N] trapezoid weights */
+} spbte_grid_dev;
+
+/* Moments of one cell used by the Maxwellian split:
+ * rho, u_x, u_y, u_z, T, prefactor rho (0.5/(pi T))^1.5. */
+#define SPBTE_MOM_STRIDE 6
+
+/* One input of the batched forward transform. mode: 0 = src as is,
+ * 1 = src - M(mom), 2 = M(mom) only (src unused).  Real input, N^3 per cell. */
+typedef struct {
+  const double *src;
+  const double *mom; /* SPBTE_MOM_STRIDE per cell, or NULL for mode 0 */
+  int mode;
+  void *dst;         /* complex output, double2 or float2 per element */
+} spbte_xform_job;
+
+/* Weighted-sum kernel arguments.  F/G arrays hold ncell * N^3 complex values
+ * (double2 for form 0, float2 otherwise); Qh gets ncell * N^3 double2. */
+typedef struct {
+  const void *W;      /* N^3 x N^3: double, float, or (hi, lo) half pairs */
+  const void *F0, *G0, *F1, *G1;
+  void *Qh;
+  int ncell;
+  int nprod;          /* 1: ComputeQ, 2: fused maxPreserve */
+} spbte_wsum_args;
+
+/* Tile of the weighted-sum kernel (see k_wsum.hip). */
+typedef struct {
+  int tr;    /* output rows (consecutive z modes) per lane */
+  int lines; /* z-lines of output modes per workgroup */
+  int tb;    /* cells per workgroup */
+} spbte_tile;
+
+/* ----- launchers (k_*.hip); all return hipError_t of the launch ----- */
+int spbte_launch_moments(const spbte_grid_dev *g, const double *f, double *mom, int ncell, hipStream_t s);
+int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform_job *jobs, int njobs, int ncell, int out_fp32,
+                           void *scratch, hipStream_t s);
+int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert,
+                                   void *scratch, hipStream_t s);
+int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, double *q, int ncell, void *scratch,
+                                hipStream_t s);
+int spbte_launch_wsum(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, hipStream_t s);
+int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, const float *rowscale,
+                             hipStream_t s);
+int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s);
+int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
+int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
+                         double *Q, int ncell, hipStream_t s);
+int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t n_rows, size_t n_cols, hipStream_t s);
+int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
+                             double *W, hipStream_t s);
+
+/* host helpers shared by the reference-compatible modules (spbte_core.c) */
+int spbte_host_lu_factor(int n, double *A, int *pivot);
+void spbte_host_lu_solve(int n, const double *LU, const int *pivot, double *b);
+void spbte_host_cct(int N, const double *v, int Ns, const double *masses, double *CCt);
+struct spbte_op;
+hipStream_t spbte_op_stream(struct spbte_op *op);
+const spbte_grid_dev *spbte_op_grid(struct spbte_op *op);
+int spbte_op_get_weights_rows(struct spbte_op *op, double **rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

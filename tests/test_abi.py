@@ -1,0 +1,105 @@
+"""The C-ABI library: it loads, it exports exactly the functions include/*.h
+declares, and its host-only paths (weight-file loading, ghat/gHat3) agree with
+the oracle.  No GPU compute here."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M):
+            if not m.group(0).lstrip().startswith("typedef"):
+                names.add(m.group(1))
+    return names
+
+
+def exported_functions(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {ln.split()[2] for ln in out.splitlines() if len(ln.split()) == 3 and ln.split()[1] == "T"}
+
+
+def test_library_exports_match_headers(spb):
+    spb.lib()
+    decl = declared_functions()
+    assert {"ComputeQ", "ComputeQ_maxPreserve", "conserveAllMoments", "initialize_weights", "spbte_compute_q"} <= decl
+    assert exported_functions(spb.LIB_PATH) == decl
+
+
+def test_reference_abi_signatures(spb):
+    """Every reference entry point resolves with the reference's arity
+    (src/collisions.h:27-44, src/conserve.h:6-18, src/weights.h:12-49)."""
+    L = spb.lib()
+    for name in ["initialize_coll", "dealloc_coll", "ComputeQ", "ComputeQ_maxPreserve", "fft3D", "ifft3D",
+                 "initialize_conservation", "initialize_conservation_fast", "dealloc_conservation", "factorMatrixIntoLU",
+                 "solveWithCCt", "conserveAllMoments", "createCCtAndPivot", "alloc_weights", "initialize_weights",
+                 "generate_conv_weights_iso", "dealloc_weights", "ghat", "gHat3"]:
+        assert hasattr(L, name)
+
+
+def test_species_struct_layout(spb):
+    """src/species.h:11-26 passed by value: same size and offsets as the C header."""
+    S = spb.Species
+    assert ctypes.sizeof(S) == 8 * 3 + 8 * 8 + 8 * 2 + 80
+    assert S.mass.offset == 32 and S.d_ref.offset == 48 and S.name.offset == 104
+
+
+def test_no_gpu_is_an_error_not_a_fallback(spb):
+    """Without a visible GPU the batched interface refuses to run (no CPU path)."""
+    if spb.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(spb.SpbteError):
+        spb.Operator(8, 5.0, np.linspace(-5, 5, 8), np.linspace(-3, 3, 8))
+
+
+def test_initialize_weights_loads_reference_file_and_ghat3(spb, orc, tmp_path):
+    """initialize_weights (src/weights.c:33-88) finds Weights/N8_isotropic_L_v5_lambda0.wts
+    in the working directory and loads it into the caller's rows without a GPU;
+    gHat3 (src/weights.c:181-262) then matches the oracle's integral."""
+    wd = tmp_path / "run"
+    (wd / "Weights").mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), wd / "Weights")
+    v, eta = orc.grid_0d(8, 5.0)
+    L = spb.lib()
+    cw = ctypes.POINTER(ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))()
+    L.alloc_weights(8, ctypes.byref(cw), 1)
+    eta_c = np.ascontiguousarray(eta)
+    L.initialize_weights.restype = None
+    L.initialize_weights.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, spb.Species, spb.Species]
+    cwd = os.getcwd()
+    os.chdir(wd)
+    try:
+        sp = spb.default_species()
+        L.initialize_weights(8, eta_c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5.0, 0.0, 0, 0,
+                             ctypes.cast(cw[0], ctypes.c_void_p), sp, sp)
+    finally:
+        os.chdir(cwd)
+    rows = cw[0]
+    Wf = orc.weights_load(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), 8)
+    for r in (0, 77, 511):
+        got = np.ctypeslib.as_array(rows[r], shape=(512,))
+        assert np.array_equal(got, Wf[r])
+    # gHat3 for a few (xi, zeta): prefactor * I(mu|zeta|, |xi|, |xi - mu zeta|)
+    P = 16 * np.pi ** 2 * (eta[1] - eta[0]) ** 3 / (2 * np.pi) ** 1.5 / (4 * np.pi)
+    for (l, m, n), (i, j, k) in [((0, 1, 2), (3, 4, 5)), ((7, 7, 7), (0, 0, 0)), ((4, 4, 4), (4, 4, 4))]:
+        xi = np.array([eta[l], eta[m], eta[n]]); ze = np.array([eta[i], eta[j], eta[k]])
+        ref = P * orc.ghat_integral(0.5 * np.linalg.norm(ze), np.linalg.norm(xi), np.linalg.norm(xi - 0.5 * ze), 5.0, 0.0)
+        got = L.gHat3(*xi, *ze)
+        assert abs(got - ref) <= 1e-14 * max(abs(ref), 1e-300) + 1e-18
+        w = np.ones(8); w[0] = w[-1] = 0.5
+        r_idx = (i * 8 + j) * 8 + k; c_idx = (l * 8 + m) * 8 + n
+        assert abs(w[l] * w[m] * w[n] * got - Wf[r_idx, c_idx]) <= 1e-13 * np.abs(Wf).max()
+    L.dealloc_weights(8, rows)

@@ -1,0 +1,272 @@
+"""GPU parity: every kernel of the MI355X operator against the CPU oracle on the
+same seeded inputs, through the C ABI.  Tolerances: FP64 paths 1e-12 relative
+L2 (the north-star parity bar; transforms 1e-13); reduced-precision forms 1e-6
+relative L2 against the FP64 result; conservation |C.Q| <= 1e-13 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(orc, N, Lv, kind):
+    return (orc.grid_0d if kind == "0d" else orc.grid_1d)(N, Lv)
+
+
+def _two_maxwellians(orc, v, rng):
+    """Seeded sum of two Maxwellians with unit density (the ensemble workload's initial state)."""
+    f = 0
+    ws = rng.uniform(0.2, 0.8, 2)
+    for w in ws:
+        f = f + w * orc.maxwellian(v, 1.0, rng.uniform(-1.5, 1.5, 3), rng.uniform(0.5, 2.0))
+    return f / ws.sum()
+
+
+def _cmoments(N, v, Q):
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    dv = v[1] - v[0]
+    I, J, K = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    p = (w[I] * w[J] * w[K] * dv ** 3).ravel()
+    C = np.stack([p, p * v[I].ravel(), p * v[J].ravel(), p * v[K].ravel(),
+                  p * 0.5 * (v[I] ** 2 + v[J] ** 2 + v[K] ** 2).ravel()])
+    Q = np.asarray(Q).reshape(-1, N ** 3)
+    return Q @ C.T, np.abs(Q) @ np.abs(C.T)
+
+
+# ------------------------------------------------------------------ transforms
+@pytest.mark.parametrize("N,Lv,kind", [(8, 5.0, "0d"), (16, 5.0, "0d"), (24, 6.0, "0d"), (32, 8.0, "1d"), (12, 7.0, "1d")])
+@pytest.mark.parametrize("invert", [0, 1])
+def test_transform(spb, orc, N, Lv, kind, invert):
+    v, eta = _grid(orc, N, Lv, kind)
+    rng = np.random.default_rng(N + 100 * invert)
+    x = rng.standard_normal((3, N, N, N)) + 1j * rng.standard_normal((3, N, N, N))
+    op = spb.Operator(N, Lv, v, eta)
+    y = op.transform(x, invert=invert)
+    for b in range(3):
+        assert rel_l2(y[b], orc.fft3d(N, v, eta, x[b], invert)) < 1e-13
+
+
+def test_fft3D_abi(spb, orc):
+    """fft3D (src/collisions.h:42) through the reference interface, in == out allowed."""
+    N, Lv = 16, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    spb.initialize_coll(N, Lv, v, eta)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    assert rel_l2(spb.fft3D(x, 0), orc.fft3d(N, v, eta, x, 0)) < 1e-13
+    assert rel_l2(spb.fft3D(x, 1), orc.fft3d(N, v, eta, x, 1)) < 1e-13
+    spb.dealloc_coll()
+
+
+# ------------------------------------------------------------- weighted sum
+@pytest.mark.parametrize("N,ncell", [(8, 2), (16, 3), (24, 3), (12, 2)])
+def test_qhat_fp64_random(spb, orc, N, ncell):
+    """Random W, F, G (ragged cell count vs the tile's cells per workgroup)."""
+    v, eta = orc.grid_0d(N, 5.0)
+    rng = np.random.default_rng(N)
+    W = rng.standard_normal((N ** 3, N ** 3))
+    F = rng.standard_normal((ncell, N, N, N)) + 1j * rng.standard_normal((ncell, N, N, N))
+    G = rng.standard_normal((ncell, N, N, N)) + 1j * rng.standard_normal((ncell, N, N, N))
+    op = spb.Operator(N, 5.0, v, eta)
+    op.set_weights(W)
+    Q = op.qhat(F, G, "fp64")
+    for b in range(ncell):
+        assert rel_l2(Q[b], orc.qhat(N, W, F[b], G[b])) < 1e-12
+
+
+@pytest.mark.parametrize("N", [16, 24])
+def test_qhat_tiles_agree(spb, orc, N):
+    """Every instantiated tile gives the same FP64 sum (index map of each lane layout)."""
+    v, eta = orc.grid_0d(N, 5.0)
+    rng = np.random.default_rng(5)
+    W = rng.standard_normal((N ** 3, N ** 3))
+    ncell = 5
+    F = rng.standard_normal((ncell, N ** 3)) + 1j * rng.standard_normal((ncell, N ** 3))
+    G = rng.standard_normal((ncell, N ** 3)) + 1j * rng.standard_normal((ncell, N ** 3))
+    op = spb.Operator(N, 5.0, v, eta)
+    op.set_weights(W)
+    ref = np.stack([orc.qhat(N, W, F[b], G[b]).ravel() for b in range(ncell)])
+    tiles = {16: [(4, 4, 1), (4, 4, 2), (4, 8, 2), (4, 8, 4)], 24: [(6, 8, 1), (6, 8, 2), (6, 8, 4)]}[N]
+    for t in tiles:
+        op.set_tile("fp64", False, *t)
+        assert rel_l2(op.qhat(F, G, "fp64"), ref) < 1e-12, t
+
+
+# --------------------------------------------------- reference interface
+def test_computeq_abi_bkw8(spb, orc):
+    """ComputeQ and ComputeQ_maxPreserve through src/collisions.h with the BKW8 table."""
+    N, Lv = 8, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_load(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), N)
+    f = orc.bkw_initial(N, v).ravel()
+    spb.initialize_coll(N, Lv, v, eta)
+    rows = spb.WeightRows(W)
+    assert rel_l2(spb.ComputeQ(f, f, rows), orc.compute_q(N, v, eta, W, f, f)) < 1e-12
+    assert rel_l2(spb.ComputeQ_maxPreserve(f, f, rows), orc.compute_q_maxpreserve(N, v, eta, W, f, f)) < 1e-12
+    # two different arguments: the reference subtracts f's Maxwellian from g (src/collisions.c:104)
+    g = orc.maxwellian(v, 0.9, np.array([0.3, 0, -0.2]), 1.2).ravel()
+    assert rel_l2(spb.ComputeQ(f, g, rows), orc.compute_q(N, v, eta, W, f, g)) < 1e-12
+    assert rel_l2(spb.ComputeQ_maxPreserve(f, g, rows), orc.compute_q_maxpreserve(N, v, eta, W, f, g)) < 1e-12
+    spb.dealloc_coll()
+
+
+def test_conserve_abi(spb, orc):
+    """conserveAllMoments (src/conserve.h:16), one and two species."""
+    N, Lv = 8, 9.0
+    v, eta = orc.grid_1d(N, Lv)
+    rng = np.random.default_rng(2)
+    Q = rng.standard_normal(N ** 3)
+    spb.initialize_conservation(N, v[1] - v[0], v)
+    (P,) = spb.conserveAllMoments([Q])
+    assert rel_l2(P, orc.Conserver(N, v)(Q)) < 1e-12
+    Cq, scale = _cmoments(N, v, P)
+    assert np.abs(Cq).max() < 1e-13 * scale.max()
+    spb.dealloc_conservation()
+    masses = (1.0, 2.0)
+    Q4 = rng.standard_normal((4, N ** 3))
+    spb.initialize_conservation(N, v[1] - v[0], v, masses)
+    P4 = np.array(spb.conserveAllMoments(list(Q4)))
+    assert rel_l2(P4, orc.Conserver(N, v, masses)(Q4)) < 1e-12
+    spb.dealloc_conservation()
+
+
+def test_bkw8_run_abi(spb, orc):
+    """The BKW8 test of the reference (tests/BKW8) driven through the GPU library:
+    100 Heun steps of ComputeQ_maxPreserve + conserveAllMoments."""
+    N, Lv, dt = 8, 5.0, 0.01
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_load(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), N)
+    spb.initialize_coll(N, Lv, v, eta)
+    spb.initialize_conservation(N, v[1] - v[0], v)
+    rows = spb.WeightRows(W)
+    f = orc.bkw_initial(N, v).ravel()
+    out = [orc.moments_row_0d(N, v, f, 0.0)]
+    for t in range(100):
+        (Q,) = spb.conserveAllMoments([spb.ComputeQ_maxPreserve(f, f, rows)])
+        f1 = f + dt * Q
+        (Q,) = spb.conserveAllMoments([spb.ComputeQ_maxPreserve(f1, f1, rows)])
+        f = 0.5 * (f + f1) + 0.5 * dt * Q
+        out.append(orc.moments_row_0d(N, v, f, dt * (t + 1)))
+    spb.dealloc_coll()
+    spb.dealloc_conservation()
+    out = np.array(out)
+    ref = np.loadtxt(os.path.join(GOLDEN, "BKW8", "moments_BKW8.test.in"), comments="#")
+    m = np.abs(ref) > 1e-12
+    assert (np.abs(out - ref)[m] / np.abs(ref)[m]).max() < 5e-7
+    orc_rows = np.array(orc.run_0d(N, v, eta, W, orc.bkw_initial(N, v), dt, 100, order=2))
+    m = np.abs(orc_rows) > 1e-12
+    assert (np.abs(out - orc_rows)[m] / np.abs(orc_rows)[m]).max() < 1e-10
+
+
+def test_heat_run_batched(spb, orc):
+    """tests/heat_transport of the reference: the collision step of all 250 cells in
+    one batched call (ComputeQ + projection), transport by the oracle, 10 steps."""
+    N, Lv, nX, dt, Kn = 8, 9.0, 250, 1e-4, 3.2
+    v, eta = orc.grid_1d(N, Lv)
+    W = orc.weights_load(os.path.join(GOLDEN, "heat_transport", "N8_isotropic_L_v9_lambda1.wts"), N)
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_weights(W)
+    x, dx = orc.uniform_mesh(nX, 1.0, 1)
+    f = np.zeros((nX + 2, N ** 3))
+    f[1:nX + 1] = orc.maxwellian_1d(v, 1.0, 0.0, 1.5).ravel()
+    snaps = []
+    for t in range(10):
+        f, fc = orc.upwind_one(N, nX, v, dx, dt, 1, f)
+        Q = op.compute_q(fc[1:nX + 1], conserve=True)
+        f[1:nX + 1] = fc[1:nX + 1] + dt * Q / Kn
+        snap = np.zeros((nX, 4))
+        for l in range(nX):
+            c = f[1 + l]
+            rho = orc.density(N, v, c); u = orc.bulk_velocity(N, v, c, rho); T = orc.temperature(N, v, c, u, rho)
+            snap[l] = (rho, u[0], T, rho * T)
+        snaps.append(snap)
+    ref = np.loadtxt(os.path.join(GOLDEN, "heat_transport", "moments_heat_transport.test.in"),
+                     comments="#").reshape(11, nX, 6)[1:]
+    snaps = np.array(snaps)
+    for c in (0, 2, 3):
+        assert (np.abs(ref[:, :, 2 + c] - snaps[:, :, c]) / np.abs(ref[:, :, 2 + c])).max() < 5e-7
+
+
+# ------------------------------------------------------------ batched path
+@pytest.mark.parametrize("N,Lv,lam,ncell", [(16, 5.0, 0.0, 5), (24, 6.0, 1.0, 3)])
+def test_batched_compute_q_fp64(spb, orc, N, Lv, lam, ncell):
+    """Batched ComputeQ / maxPreserve (+ projection) on physical inputs vs the oracle,
+    W generated on the GPU and checked against the oracle's table."""
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(lam)
+    W = op.get_weights()
+    Wo = orc.weights_iso(N, eta, Lv, lam)
+    assert np.abs(W - Wo).max() < 1e-13 * np.abs(Wo).max()
+    rng = np.random.default_rng(N)
+    f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(ncell)])
+    g = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(ncell)])
+    Q = op.compute_q(f)
+    Qg = op.compute_q(f, g)
+    Qmp = op.compute_q(f, maxpreserve=True, conserve=True)
+    Qmpg = op.compute_q(f, g, maxpreserve=True)
+    cons = orc.Conserver(N, v)
+    for b in range(ncell):
+        assert rel_l2(Q[b], orc.compute_q(N, v, eta, W, f[b], f[b])) < 1e-12
+        assert rel_l2(Qg[b], orc.compute_q(N, v, eta, W, f[b], g[b])) < 1e-12
+        ref = cons(orc.compute_q_maxpreserve(N, v, eta, W, f[b], f[b]).ravel())
+        assert rel_l2(Qmp[b], ref) < 1e-12
+        assert rel_l2(Qmpg[b], orc.compute_q_maxpreserve(N, v, eta, W, f[b], g[b])) < 1e-12
+    Cq, scale = _cmoments(N, v, Qmp)
+    assert np.abs(Cq).max() < 1e-13 * scale.max()
+
+
+def test_weights_generation_matches_fixtures(spb, orc):
+    """generate_conv_weights_iso on the GPU reproduces both reference tables."""
+    for kind, Lv, lam, path in [("0d", 5.0, 0.0, "BKW8/N8_isotropic_L_v5_lambda0.wts"),
+                                ("1d", 9.0, 1.0, "heat_transport/N8_isotropic_L_v9_lambda1.wts")]:
+        v, eta = _grid(orc, 8, Lv, kind)
+        op = spb.Operator(8, Lv, v, eta)
+        op.generate_weights_iso(lam)
+        Wf = orc.weights_load(os.path.join(GOLDEN, path), 8)
+        assert np.abs(op.get_weights() - Wf).max() <= 2e-15 * np.abs(Wf).max()
+    v, eta = orc.grid_1d(8, 6.0)
+    op = spb.Operator(8, 6.0, v, eta)
+    op.generate_weights_iso(0.5)
+    Wo = orc.weights_iso(8, eta, 6.0, 0.5)
+    assert np.abs(op.get_weights() - Wo).max() < 1e-12 * np.abs(Wo).max()
+
+
+@pytest.mark.parametrize("N,Lv", [(16, 5.0), (24, 6.0)])
+def test_reduced_precision_forms(spb, orc, N, Lv):
+    """FP32 and FP16x2 streamed weights: <= 1e-6 relative L2 vs FP64, projection exact."""
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    rng = np.random.default_rng(11)
+    f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(4)])
+    ref = op.compute_q(f, conserve=True)
+    for form in ("fp32", "fp16x2"):
+        for mp in (False, True):
+            q = op.compute_q(f, form=form, maxpreserve=mp, conserve=True)
+            r = op.compute_q(f, maxpreserve=mp, conserve=True) if mp else ref
+            err = max(rel_l2(q[b], r[b]) for b in range(4))
+            assert err < 1e-6, (form, mp, err)
+            Cq, scale = _cmoments(N, v, q)
+            assert np.abs(Cq).max() < 1e-13 * scale.max()
+
+
+def test_n32_single_cell_against_oracle(spb, orc):
+    """The headline grid (N=32, L_v=8, hard spheres, 1D grid): one cell, FP64 through
+    the batched path against the oracle's full O(N^6) evaluation; FP32 forms <= 1e-6."""
+    N, Lv = 32, 8.0
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    rng = np.random.default_rng(32)
+    f = _two_maxwellians(orc, v, rng).ravel()
+    q64 = op.compute_q(f[None])[0]
+    W = op.get_weights()
+    ref = orc.compute_q(N, v, eta, W, f, f)
+    del W
+    assert rel_l2(q64, ref) < 1e-12
+    for form in ("fp32", "fp16x2"):
+        assert rel_l2(op.compute_q(f[None], form=form)[0], q64) < 1e-6

@@ -1,0 +1,186 @@
+"""Independent checks of the CPU oracle: each pins one function by a route other
+than its own formula (a library routine, a closed form, an invariant)."""
+import numpy as np
+import pytest
+from scipy import integrate
+
+from conftest import rel_l2
+
+
+@pytest.mark.parametrize("N,Lv,grid", [(4, 3.0, "0d"), (8, 5.0, "0d"), (6, 4.0, "1d"), (8, 9.0, "1d")])
+@pytest.mark.parametrize("invert", [0, 1])
+def test_fft3d_matches_library_fft(orc, N, Lv, grid, invert):
+    """src/collisions.c:232-283 is pre-twiddle * FFTW * post-twiddle; rebuild it with
+    numpy's FFT (pocketfft) and compare with the oracle's direct separable DFT."""
+    v, eta = (orc.grid_0d if grid == "0d" else orc.grid_1d)(N, Lv)
+    rng = np.random.default_rng(N * 10 + invert)
+    x = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    out = orc.fft3d(N, v, eta, x, invert)
+    dv, deta = v[1] - v[0], eta[1] - eta[0]
+    if not invert:
+        delta, Ls, Le, a, s = dv, -eta[0], Lv, eta, 1.0
+    else:
+        delta, Ls, Le, a, s = deta, Lv, -eta[0], v, -1.0
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    idx = np.arange(N)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    pre = (2 * np.pi) ** -1.5 * delta ** 3 * w[I] * w[J] * w[K] * np.exp(1j * s * (I + J + K) * Ls * delta)
+    y = pre * x
+    Y = np.fft.ifftn(y) * N ** 3 if invert else np.fft.fftn(y)
+    post = np.exp(1j * s * Le * (a[I] + a[J] + a[K]))
+    assert rel_l2(out, post * Y) < 1e-14
+
+
+def test_fft3d_gaussian_closed_form(orc):
+    """Forward transform of a centred Gaussian approaches its continuous Fourier
+    transform (2 pi)^-3/2 int f e^{-i eta.v} dv = (2 pi)^-3/2 (2 pi)^3/2 e^{-|eta|^2/2}."""
+    N, Lv = 32, 10.0
+    v, eta = orc.grid_0d(N, Lv)
+    V2 = v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2
+    f = (2 * np.pi) ** -1.5 * np.exp(-V2 / 2)
+    out = orc.fft3d(N, v, eta, f.astype(complex), 0)
+    E2 = eta[:, None, None] ** 2 + eta[None, :, None] ** 2 + eta[None, None, :] ** 2
+    ref = (2 * np.pi) ** -1.5 * np.exp(-E2 / 2)
+    # trapezoid aliasing ~ exp(-(2 pi/h_v - |eta|)^2/2): below 1e-13 for |eta_d| <= 2
+    m = (np.abs(eta)[:, None, None] <= 2) & (np.abs(eta)[None, :, None] <= 2) & (np.abs(eta)[None, None, :] <= 2)
+    assert m.sum() > 100
+    assert np.abs(out - ref)[m].max() < 1e-12
+
+
+@pytest.mark.parametrize("N", [4, 6, 8])
+def test_qhat_convolution_theorem(orc, N):
+    """With W == 1, sum_c G[c] F[(r + N/2 - c) mod N] is a circular convolution
+    shifted by N/2 (src/collisions.c:141-160): check it against FFT convolution."""
+    rng = np.random.default_rng(N)
+    F = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    G = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    W = np.ones((N ** 3, N ** 3))
+    Q = orc.qhat(N, W, F, G)
+    conv = np.fft.ifftn(np.fft.fftn(G) * np.fft.fftn(F))
+    ref = np.roll(conv, shift=(-(N // 2),) * 3, axis=(0, 1, 2))
+    assert rel_l2(Q, ref) < 1e-13
+
+
+def test_qhat_single_entry_index_map(orc):
+    """W with one non-zero entry (r0, c0): only Qhat[r0] = W G[c0] F[s] is non-zero,
+    s_d = (r0_d + N/2 - c0_d) mod N, for an asymmetric (r0, c0)."""
+    N = 6
+    rng = np.random.default_rng(7)
+    F = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    G = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    r0, c0 = (1, 4, 5), (5, 0, 2)
+    ri = (r0[0] * N + r0[1]) * N + r0[2]
+    ci = (c0[0] * N + c0[1]) * N + c0[2]
+    W = np.zeros((N ** 3, N ** 3)); W[ri, ci] = 2.5
+    Q = orc.qhat(N, W, F, G)
+    s = tuple((r0[d] + N // 2 - c0[d]) % N for d in range(3))
+    expect = np.zeros((N, N, N), complex); expect[r0] = 2.5 * G[c0] * F[s]
+    mask = np.ones((N, N, N), bool); mask[r0] = False
+    assert np.abs(Q[mask]).max() == 0.0
+    assert abs(Q[r0] - expect[r0]) < 4e-16 * abs(expect[r0])
+
+
+def _cmat(N, v):
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    dv = v[1] - v[0]
+    I, J, K = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    p = (w[I] * w[J] * w[K] * dv ** 3).ravel()
+    return np.stack([p, p * v[I].ravel(), p * v[J].ravel(), p * v[K].ravel(),
+                     p * 0.5 * (v[I] ** 2 + v[J] ** 2 + v[K] ** 2).ravel()])
+
+
+@pytest.mark.parametrize("N,Lv", [(8, 5.0), (12, 7.0)])
+def test_conserve_is_the_orthogonal_projection(orc, N, Lv):
+    """src/conserve.c:207-264 with one species is Q - C^T (C C^T)^-1 C Q."""
+    v, _ = orc.grid_0d(N, Lv)
+    rng = np.random.default_rng(N)
+    Q = rng.standard_normal(N ** 3)
+    cons = orc.Conserver(N, v)
+    P = cons(Q)
+    C = _cmat(N, v)
+    ref = Q - C.T @ np.linalg.solve(C @ C.T, C @ Q)
+    assert rel_l2(P, ref) < 1e-13
+    assert np.abs(C @ P).max() < 1e-13 * np.abs(C @ np.abs(Q)).max()
+    assert rel_l2(cons(P), P) < 1e-14  # idempotent
+
+
+def test_conserve_two_species_moments(orc):
+    """Ns=2 (src/conserve.c:213-263): after projection the total mass of each
+    species and the total momentum/energy over species vanish."""
+    N, Lv = 6, 4.0
+    v, _ = orc.grid_0d(N, Lv)
+    masses = np.array([1.0, 2.0])
+    rng = np.random.default_rng(3)
+    Q = rng.standard_normal((4, N ** 3))
+    P = orc.Conserver(N, v, masses)(Q)
+    C = _cmat(N, v)
+    S = [P[0] + P[2], P[1] + P[3]]  # species m: arrays n*Ns+m
+    mass = [masses[m] * C[0] @ S[m] for m in range(2)]
+    mom_en = sum(masses[m] * C[1:] @ S[m] for m in range(2))
+    scale = np.abs(C @ np.abs(Q).sum(0)).max()
+    assert max(abs(mass[0]), abs(mass[1])) < 1e-13 * scale
+    assert np.abs(mom_en).max() < 1e-13 * scale
+
+
+def test_lu_pivot_rule(orc):
+    """src/conserve.c:115-126 takes the FIRST row whose scaled entry beats the
+    diagonal, not the largest; the factors still solve the system exactly."""
+    import ctypes
+    A = np.array([[1.0, 2.0, 0.0], [3.0, 1.0, 1.0], [9.0, 1.0, 2.0]])
+    LU = A.copy().ravel()
+    piv = np.zeros(3, dtype=np.int32)
+    assert orc.lib().orc_lu_factor(3, LU.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) == 0
+    assert piv[0] == 1  # row 1 (3/3=1 > 1/2) comes before the larger row 2 (9/9=1, ties do not count)
+    b = np.array([1.0, -2.0, 0.5])
+    x = b.copy()
+    orc.lib().orc_lu_solve(3, LU.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert np.abs(A @ x - b).max() < 1e-14
+
+
+@pytest.mark.parametrize("lam", [0.0, 1.0, 0.5, -0.5])
+def test_radial_integral_against_quadpack(orc, lam):
+    """src/weights.c:156-160,203: I = int_0^L r^(lam+2)[sinc(a0 r)sinc(a2 r) - sinc(a1 r)] dr,
+    closed forms (lam 0, 1) or Gauss-Legendre (else) against scipy's QUADPACK."""
+    import warnings
+    warnings.simplefilter("ignore", integrate.IntegrationWarning)
+    L = 7.0
+    rng = np.random.default_rng(int(lam * 10) + 5)
+    cases = [(0.0, 0.0, 0.0), (0.0, 1.3, 0.9), (0.8, 0.0, 0.0), (0.0, 0.4, 0.0), (1.1, 1.1, 1.1)]
+    cases += [tuple(rng.uniform(0, 3, 3)) for _ in range(8)]
+
+    def sinc(x):
+        return np.sinc(x / np.pi)
+
+    for a0, a1, a2 in cases:
+        ref, _ = integrate.quad(lambda r: r ** (lam + 2) * (sinc(a0 * r) * sinc(a2 * r) - sinc(a1 * r)), 0, L,
+                                epsabs=1e-14, epsrel=1e-13, limit=400)
+        got = orc.ghat_integral(a0, a1, a2, L, lam)
+        scale = L ** (lam + 3) / (lam + 3)
+        assert abs(got - ref) < 1e-12 * scale, (a0, a1, a2, got, ref)
+
+
+def test_moments_of_discrete_maxwellian(orc):
+    """src/momentRoutines.c:58-183 on a wide grid reproduce (rho, u, T) of a
+    Maxwellian to quadrature accuracy, and the KB rule (mass != 1) scales T."""
+    N, Lv = 32, 10.0
+    v, _ = orc.grid_0d(N, Lv)
+    f = orc.maxwellian(v, 1.3, np.array([0.4, -0.2, 0.1]), 0.9)
+    rho = orc.density(N, v, f)
+    u = orc.bulk_velocity(N, v, f, rho)
+    T = orc.temperature(N, v, f, u, rho)
+    assert abs(rho - 1.3) < 1e-8 and np.abs(u - [0.4, -0.2, 0.1]).max() < 1e-8 and abs(T - 0.9) < 1e-8
+
+
+def test_maxpreserve_decomposition(orc):
+    """Q(M,g)+Q(g,M)+Q(g,g) = Q(f,f) - Q(M,M) by bilinearity (src/collisions.c:178-210)."""
+    N, Lv = 8, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_iso(N, eta, Lv, 0.0)
+    f = orc.bkw_initial(N, v)
+    Qmp = orc.compute_q_maxpreserve(N, v, eta, W, f, f)
+    rho = orc.density(N, v, f); u = orc.bulk_velocity(N, v, f, rho); T = orc.temperature(N, v, f, u, rho)
+    M = orc.maxwellian(v, rho, u, T)
+    ref = orc.compute_q(N, v, eta, W, f, f) - orc.compute_q(N, v, eta, W, M, M)
+    assert np.abs(Qmp - ref).max() < 1e-12 * np.abs(ref).max()
