@@ -35,10 +35,12 @@ enum {
 };
 
 /* Precision forms of the weighted sum.  FP64 is the reference method.  FP32
- * streams W as float and sums in compensated FP32.  FP16X2 streams W as two
- * FP16 halves (hi + lo) with a power-of-two scale per row.  Transforms,
- * moments and projection are FP64 in every form. */
-enum { SPBTE_FORM_FP64 = 0, SPBTE_FORM_FP32 = 1, SPBTE_FORM_FP16X2 = 2 };
+ * streams W as float (half the HBM bytes) and widens it to FP64 in registers;
+ * FP16X2 streams W as two FP16 halves (hi + lo) with a power-of-two scale per
+ * row, widened to FP64; FP32A streams W as float and also computes the products
+ * in FP32, with compensated (TwoSum) FP32 sums.  Transforms, moments and the
+ * projection are FP64 in every form. */
+enum { SPBTE_FORM_FP64 = 0, SPBTE_FORM_FP32 = 1, SPBTE_FORM_FP16X2 = 2, SPBTE_FORM_FP32A = 3 };
 
 /* flags of spbte_compute_q */
 enum {
@@ -58,6 +60,9 @@ int spbte_free(void *d_ptr);
 int spbte_memcpy_htod(void *d_dst, const void *src, size_t bytes);
 int spbte_memcpy_dtoh(void *dst, const void *d_src, size_t bytes);
 int spbte_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
+/* Copy with a plain streaming kernel (16-byte aligned, size a multiple of 16);
+ * synchronous.  Also the known-size reference for HBM byte counters. */
+int spbte_device_copy(void *d_dst, const void *d_src, size_t bytes);
 int spbte_op_synchronize(spbte_op *op);
 
 /* One velocity grid: N nodes per axis on [-L_v, L_v] with the caller's v and

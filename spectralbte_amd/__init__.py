@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libspectralbte_amd.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
 
-FORMS = {"fp64": 0, "fp32": 1, "fp16x2": 2}
+FORMS = {"fp64": 0, "fp32": 1, "fp16x2": 2, "fp32a": 3}
 MAXPRESERVE, CONSERVE, SELF = 1, 2, 4
 
 _lib = None
@@ -81,6 +81,7 @@ def lib():
             "spbte_memcpy_htod": (I, [_vp, _vp, S]),
             "spbte_memcpy_dtoh": (I, [_vp, _vp, S]),
             "spbte_memcpy_dtod": (I, [_vp, _vp, S]),
+            "spbte_device_copy": (I, [_vp, _vp, S]),
             "spbte_op_synchronize": (I, [_vp]),
             "spbte_op_create": (I, [ctypes.POINTER(_vp), I, D, _dp, _dp]),
             "spbte_op_destroy": (None, [_vp]),

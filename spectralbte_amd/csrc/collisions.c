@@ -9,7 +9,7 @@
  * crosses PCIe once.
  *
  * The precision form defaults to FP64 (the reference method); the environment
- * variable SPBTE_FORM = fp32 | fp16x2 selects another.  HIP failures print a
+ * variable SPBTE_FORM = fp32 | fp16x2 | fp32a selects another.  HIP failures print a
  * message and exit(1), which is how the reference handles fatal errors
  * (src/weights.c:84-86).
  */
@@ -44,6 +44,7 @@ static int form_from_env(void) {
   if (!s) return SPBTE_FORM_FP64;
   if (!strcmp(s, "fp32")) return SPBTE_FORM_FP32;
   if (!strcmp(s, "fp16x2")) return SPBTE_FORM_FP16X2;
+  if (!strcmp(s, "fp32a")) return SPBTE_FORM_FP32A;
   return SPBTE_FORM_FP64;
 }
 

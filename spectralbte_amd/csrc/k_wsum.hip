@@ -25,11 +25,15 @@
 //  * blockIdx -> (r-tile, cell group) is XCD-aware: consecutive workgroups that
 //    land on one XCD (bid mod 8) share the r-tile, so the W lines they stream
 //    are shared through that XCD's L2 when several cell groups run together.
-//  * Precision forms: FP64 (W, F, G, sums all FP64: the reference method);
-//    FP32 (W, F, G in FP32, per-cx FP32 partial sums folded into a compensated
-//    (TwoSum) FP32 pair); FP16x2 (W = s_r (hi + lo), two FP16 halves per entry with
-//    a power-of-two row scale s_r, expanded to FP32 in registers).  The final
-//    cross-lane combination and the output are FP64.
+//  * Precision forms (FORM): 0 = FP64 W, FP64 arithmetic (the reference method);
+//    1 = FP32 W streamed (half the HBM bytes), widened to FP64 in registers, FP64
+//    arithmetic; 2 = FP16x2 W: W = s_r (hi + lo), two FP16 halves per entry with a
+//    power-of-two row scale s_r, widened to FP64; 3 = FP32 W and FP32 arithmetic
+//    (F, G, products in FP32), partial sums over 4 cy folded into a compensated
+//    (TwoSum) FP32 pair.  On CDNA4 an FP64 FMA issues at the same vector rate as an
+//    unpacked FP32 FMA, so forms 1-2 cost the same VALU as form 3 and are accurate
+//    to the rounding of W alone; form 3 halves LDS traffic and registers instead.
+//    The final cross-lane combination and the output are FP64 in every form.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "spbte_internal.h"
@@ -58,31 +62,41 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float x) {
   lo += e;
 }
 
-// Load 4 consecutive weights of one row as CT.
+// Load 4 consecutive weights of one row (storage WF) as CT.
 template <int WF, typename CT>
 __device__ __forceinline__ void load_w4(const void *p, CT (&w)[4]) {
   if constexpr (WF == 0) {
     const double2 *q = (const double2 *)p;
     const double2 a = q[0], b = q[1];
-    w[0] = a.x; w[1] = a.y; w[2] = b.x; w[3] = b.y;
+    w[0] = (CT)a.x; w[1] = (CT)a.y; w[2] = (CT)b.x; w[3] = (CT)b.y;
   } else if constexpr (WF == 1) {
     const float4 a = *(const float4 *)p;
-    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+    w[0] = (CT)a.x; w[1] = (CT)a.y; w[2] = (CT)a.z; w[3] = (CT)a.w;
   } else {
     const uint4 a = *(const uint4 *)p;
     const unsigned u[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const h2v hl = __builtin_bit_cast(h2v, u[j]);  // .x = hi, .y = lo
-      w[j] = (float)hl.x + (float)hl.y;
+      w[j] = (CT)hl.x + (CT)hl.y;
     }
   }
 }
 
-template <int N, int TR, int LINES, int TB, int NP, int WF>
+// FORM -> (W storage, arithmetic): see the header comment.
+template <int FORM> struct FormT;
+template <> struct FormT<0> { static constexpr int WF = 0; using CT = double; };
+template <> struct FormT<1> { static constexpr int WF = 1; using CT = double; };
+template <> struct FormT<2> { static constexpr int WF = 2; using CT = double; };
+template <> struct FormT<3> { static constexpr int WF = 1; using CT = float; };
+
+template <int N, int TR, int LINES, int TB, int NP, int FORM>
 struct Cfg {
-  using CT = typename std::conditional<WF == 0, double, float>::type;
+  static constexpr int WF = FormT<FORM>::WF;
+  using CT = typename FormT<FORM>::CT;
   using C2 = typename cplx<CT>::t;
+  static constexpr bool A32 = std::is_same<CT, float>::value;
+  static constexpr int FOLD = 4;              // cy per compensated fold (FP32 arithmetic)
   static constexpr int NCC = N / 4;           // cz chunks of 4
   static constexpr int NRG = N / TR;          // row groups per z-line
   static constexpr int PER_LINE = NCC * NRG;  // lanes per z-line
@@ -97,10 +111,11 @@ struct Cfg {
   static_assert(N % 4 == 0 && N % TR == 0 && N % LINES == 0, "tile must divide N");
 };
 
-template <int N, int TR, int LINES, int TB, int NP, int WF>
-__global__ void __launch_bounds__((Cfg<N, TR, LINES, TB, NP, WF>::NT))
+template <int N, int TR, int LINES, int TB, int NP, int FORM>
+__global__ void __launch_bounds__((Cfg<N, TR, LINES, TB, NP, FORM>::NT))
 k_wsum(KArgs a) {
-  using C = Cfg<N, TR, LINES, TB, NP, WF>;
+  using C = Cfg<N, TR, LINES, TB, NP, FORM>;
+  constexpr int WF = C::WF;
   using CT = typename C::CT;
   using C2 = typename C::C2;
   constexpr int NCC = C::NCC, NRG = C::NRG, PER_LINE = C::PER_LINE, NT = C::NT;
@@ -138,9 +153,9 @@ k_wsum(KArgs a) {
   const C2 *Gp[2] = {(const C2 *)a.G0, (const C2 *)a.G1};
 
   // accumulators
-  using A2 = typename std::conditional<WF == 0, double2, float2>::type;
+  using A2 = C2;
   A2 acc[TB][TR];
-  float2 chi[TB][TR], clo[TB][TR];  // compensated pairs (FP32 forms only)
+  float2 chi[TB][TR], clo[TB][TR];  // compensated pairs (FP32 arithmetic only)
 #pragma unroll
   for (int b = 0; b < TB; ++b)
 #pragma unroll
@@ -224,17 +239,19 @@ k_wsum(KArgs a) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) wcur[t][j] = wnxt[t][j];
       }
-    }
-    if constexpr (WF != 0) {
+      if constexpr (C::A32) {
+        if ((cy % C::FOLD) == C::FOLD - 1) {
 #pragma unroll
-      for (int b = 0; b < TB; ++b)
+          for (int b = 0; b < TB; ++b)
 #pragma unroll
-        for (int t = 0; t < TR; ++t) {
-          two_sum(chi[b][t].x, clo[b][t].x, acc[b][t].x);
-          two_sum(chi[b][t].y, clo[b][t].y, acc[b][t].y);
-          acc[b][t].x = 0.f;
-          acc[b][t].y = 0.f;
+            for (int t = 0; t < TR; ++t) {
+              two_sum(chi[b][t].x, clo[b][t].x, acc[b][t].x);
+              two_sum(chi[b][t].y, clo[b][t].y, acc[b][t].y);
+              acc[b][t].x = 0.f;
+              acc[b][t].y = 0.f;
+            }
         }
+      }
     }
   }
 
@@ -246,7 +263,7 @@ k_wsum(KArgs a) {
 #pragma unroll
     for (int t = 0; t < TR; ++t) {
       double2 v;
-      if constexpr (WF == 0) v = acc[b][t];
+      if constexpr (!C::A32) v = acc[b][t];
       else {
         v.x = (double)chi[b][t].x + (double)clo[b][t].x;
         v.y = (double)chi[b][t].y + (double)clo[b][t].y;
@@ -264,7 +281,7 @@ k_wsum(KArgs a) {
           s.x += o.x;
           s.y += o.y;
         }
-        if constexpr (WF == 2) {
+        if constexpr (C::WF == 2) {
           const double sc = (double)a.rowscale[row0 + t];
           s.x *= sc;
           s.y *= sc;
@@ -318,9 +335,9 @@ k_wsum_generic(int N, const double *__restrict__ W, const double2 *__restrict__ 
   Qh[cell * n3 + r] = make_double2(qr, qi);
 }
 
-template <int N, int TR, int LINES, int TB, int NP, int WF>
+template <int N, int TR, int LINES, int TB, int NP, int FORM>
 int launch_t(const spbte_wsum_args *a, const float *rowscale, hipStream_t s) {
-  using C = Cfg<N, TR, LINES, TB, NP, WF>;
+  using C = Cfg<N, TR, LINES, TB, NP, FORM>;
   KArgs k;
   k.W = a->W;
   k.rowscale = rowscale;
@@ -329,41 +346,43 @@ int launch_t(const spbte_wsum_args *a, const float *rowscale, hipStream_t s) {
   k.ncell = a->ncell;
   const int ncg = (a->ncell + TB - 1) / TB;
   const int nrt = N * (N / LINES);
-  hipLaunchKernelGGL((k_wsum<N, TR, LINES, TB, NP, WF>), dim3(nrt * ncg), dim3(C::NT), 0, s, k);
+  hipLaunchKernelGGL((k_wsum<N, TR, LINES, TB, NP, FORM>), dim3(nrt * ncg), dim3(C::NT), 0, s, k);
   return (int)hipGetLastError();
 }
 
 // ---- instantiation table: (N, TR, LINES, TB) per form and product count
 typedef int (*launch_fn)(const spbte_wsum_args *, const float *, hipStream_t);
 struct Entry {
-  int N, tr, lines, tb, np, wf;
+  int N, tr, lines, tb, np, form;
   launch_fn fn;
 };
 
-#define E(N, TR, L, TB, NP, WF) {N, TR, L, TB, NP, WF, &launch_t<N, TR, L, TB, NP, WF>}
-#define E_ALLFORMS(N, TR, L, TB, NP) E(N, TR, L, TB, NP, 0), E(N, TR, L, TB, NP, 1), E(N, TR, L, TB, NP, 2)
+#define E(N, TR, L, TB, NP, F) {N, TR, L, TB, NP, F, &launch_t<N, TR, L, TB, NP, F>}
+#define E_ALLFORMS(N, TR, L, TB, NP) E(N, TR, L, TB, NP, 0), E(N, TR, L, TB, NP, 1), E(N, TR, L, TB, NP, 2), \
+                                     E(N, TR, L, TB, NP, 3)
 
 const Entry kTable[] = {
-    // N = 32: 64 lanes per z-line (TR=4)
-    E_ALLFORMS(32, 4, 1, 1, 1), E_ALLFORMS(32, 4, 1, 2, 1), E_ALLFORMS(32, 4, 2, 1, 1), E_ALLFORMS(32, 4, 2, 2, 1),
-    E_ALLFORMS(32, 4, 4, 1, 1), E_ALLFORMS(32, 4, 4, 2, 1), E_ALLFORMS(32, 4, 2, 4, 1), E_ALLFORMS(32, 4, 4, 4, 1),
+    // N = 32: 64 lanes per z-line at TR=4, 32 at TR=8
+    E_ALLFORMS(32, 4, 1, 1, 1), E_ALLFORMS(32, 4, 2, 1, 1), E_ALLFORMS(32, 4, 2, 2, 1), E_ALLFORMS(32, 4, 4, 2, 1),
+    E_ALLFORMS(32, 4, 2, 4, 1), E_ALLFORMS(32, 4, 4, 4, 1), E_ALLFORMS(32, 8, 4, 2, 1), E_ALLFORMS(32, 8, 8, 2, 1),
+    E_ALLFORMS(32, 8, 4, 4, 1),
     E_ALLFORMS(32, 4, 1, 1, 2), E_ALLFORMS(32, 4, 2, 1, 2), E_ALLFORMS(32, 4, 2, 2, 2), E_ALLFORMS(32, 4, 4, 2, 2),
-    E_ALLFORMS(32, 8, 4, 2, 1), E_ALLFORMS(32, 8, 8, 2, 1), E_ALLFORMS(32, 8, 8, 4, 1), E_ALLFORMS(32, 8, 4, 4, 1),
     // N = 24: TR=6 -> 24 lanes per line
     E_ALLFORMS(24, 6, 8, 1, 1), E_ALLFORMS(24, 6, 8, 2, 1), E_ALLFORMS(24, 6, 8, 4, 1),
     E_ALLFORMS(24, 6, 8, 1, 2), E_ALLFORMS(24, 6, 8, 2, 2),
     // N = 16: TR=4 -> 16 lanes per line
     E_ALLFORMS(16, 4, 4, 1, 1), E_ALLFORMS(16, 4, 4, 2, 1), E_ALLFORMS(16, 4, 8, 2, 1), E_ALLFORMS(16, 4, 8, 4, 1),
     E_ALLFORMS(16, 4, 4, 1, 2), E_ALLFORMS(16, 4, 4, 2, 2), E_ALLFORMS(16, 4, 8, 2, 2),
-    // N = 8: TR=2 -> 8 lanes per line (FP32 forms; FP64 N=8 runs the generic kernel)
-    E(8, 2, 8, 2, 2, 1), E(8, 2, 8, 2, 2, 2), E(8, 2, 8, 2, 1, 1), E(8, 2, 8, 2, 1, 2),
+    // N = 8: TR=2 -> 8 lanes per line (reduced forms; FP64 N=8 runs the generic kernel)
+    E(8, 2, 8, 2, 1, 1), E(8, 2, 8, 2, 1, 2), E(8, 2, 8, 2, 1, 3), E(8, 2, 8, 2, 2, 1), E(8, 2, 8, 2, 2, 2),
+    E(8, 2, 8, 2, 2, 3),
 };
 #undef E_ALLFORMS
 #undef E
 
 const Entry *find(int N, int form, int nprod, const spbte_tile *t) {
   for (const Entry &e : kTable)
-    if (e.N == N && e.wf == form && e.np == nprod && e.tr == t->tr && e.lines == t->lines && e.tb == t->tb) return &e;
+    if (e.N == N && e.form == form && e.np == nprod && e.tr == t->tr && e.lines == t->lines && e.tb == t->tb) return &e;
   return nullptr;
 }
 
@@ -452,5 +471,19 @@ __global__ void k_c64_to_c32(const double2 *__restrict__ in, float2 *__restrict_
 
 extern "C" int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_c64_to_c32, dim3(1024), dim3(256), 0, s, (const double2 *)in, (float2 *)out, n);
+  return (int)hipGetLastError();
+}
+
+// ---- plain device copy (also the reference kernel for calibrating HBM byte counters)
+namespace {
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ in, uint4 *__restrict__ out, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+}  // namespace
+
+extern "C" int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s) {
+  if (bytes % 16 || ((uintptr_t)in | (uintptr_t)out) % 16) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s, (const uint4 *)in, (uint4 *)out, bytes / 16);
   return (int)hipGetLastError();
 }

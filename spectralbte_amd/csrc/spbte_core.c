@@ -44,7 +44,7 @@ static int fail(int code, const char *fmt, ...) {
   } while (0)
 
 /* --------------------------------------------------------------- op state */
-#define NFORMS 3
+#define NFORMS 4
 #define MAXJOBS 4
 
 struct spbte_op {
@@ -55,7 +55,7 @@ struct spbte_op {
   hipStream_t s;
   /* weight table: FP64 master copy and derived forms */
   double *dW64;
-  void *dWf[NFORMS]; /* [0] aliases dW64; [1] float; [2] (hi,lo) pairs + row scales */
+  void *dWf[NFORMS]; /* [0] aliases dW64; [1] float (forms 1, 3); [2] (hi,lo) pairs + row scales */
   int have_w;
   /* conservation */
   int Ns;
@@ -96,6 +96,11 @@ int spbte_memcpy_dtoh(void *d, const void *s, size_t b) {
 }
 int spbte_memcpy_dtod(void *d, const void *s, size_t b) {
   HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToDevice));
+  return SPBTE_OK;
+}
+int spbte_device_copy(void *d_dst, const void *d_src, size_t bytes) {
+  LCHK(spbte_launch_copy(d_src, d_dst, bytes, 0));
+  HIPCHK(hipStreamSynchronize(0));
   return SPBTE_OK;
 }
 int spbte_op_synchronize(spbte_op *op) {
@@ -420,21 +425,28 @@ int spbte_op_get_weights_rows(spbte_op *op, double **rows) {
   return SPBTE_OK;
 }
 
+/* Storage of the table each form streams: FP64, FP32 (forms 1 and 3) or FP16x2. */
+static int storage_of(int form) { return form == SPBTE_FORM_FP32A ? SPBTE_FORM_FP32 : form; }
+
+/* F/G element size of a form: FP32 complex only when the arithmetic is FP32. */
+static size_t fg_bytes(int form) { return form == SPBTE_FORM_FP32A ? 8 : 16; }
+
 static int weights_form(spbte_op *op, int form, const void **W, const float **rowscale) {
   if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   *rowscale = NULL;
-  if (form == SPBTE_FORM_FP64) {
+  const int st = storage_of(form);
+  if (st == SPBTE_FORM_FP64) {
     *W = op->dW64;
     return SPBTE_OK;
   }
   const size_t n = op->n3 * op->n3;
-  if (!op->dWf[form]) {
-    size_t bytes = (form == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
-    HIPCHK(hipMalloc(&op->dWf[form], bytes));
-    LCHK(spbte_launch_convert_weights(op->dW64, op->dWf[form], form, op->n3, op->n3, op->s));
+  if (!op->dWf[st]) {
+    size_t bytes = (st == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
+    HIPCHK(hipMalloc(&op->dWf[st], bytes));
+    LCHK(spbte_launch_convert_weights(op->dW64, op->dWf[st], st, op->n3, op->n3, op->s));
   }
-  *W = op->dWf[form];
-  if (form == SPBTE_FORM_FP16X2) *rowscale = (const float *)((const unsigned *)op->dWf[form] + n);
+  *W = op->dWf[st];
+  if (st == SPBTE_FORM_FP16X2) *rowscale = (const float *)((const unsigned *)op->dWf[st] + n);
   return SPBTE_OK;
 }
 
@@ -461,7 +473,7 @@ static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   switch (N) {
     case 32:
       if (nprod == 2) { c = c32_2; nc = 4; }
-      else if (form == 0) { c = c32_1_64; nc = 4; }
+      else if (form != SPBTE_FORM_FP32A) { c = c32_1_64; nc = 4; }
       else { c = c32_1; nc = 6; }
       break;
     case 24: if (nprod == 2) { c = c24_2; nc = 2; } else { c = c24_1; nc = 3; } break;
@@ -516,7 +528,7 @@ static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const voi
   spbte_tile t;
   int have = pick_tile(op, ncell, form, nprod, &t);
   if (!have && form != SPBTE_FORM_FP64)
-    return fail(SPBTE_ERR_UNSUPPORTED, "no FP32/FP16x2 weighted-sum kernel for N=%d", op->N);
+    return fail(SPBTE_ERR_UNSUPPORTED, "no reduced-precision weighted-sum kernel for N=%d", op->N);
   spbte_wsum_args a = {W, F0, G0, F1, G1, op->d_Qh, ncell, nprod};
   LCHK(spbte_launch_wsum_scaled(op->N, form, have ? &t : NULL, &a, rs, op->s));
   return SPBTE_OK;
@@ -529,7 +541,7 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
   const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
   const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
   const size_t n = (size_t)ncell * op->n3;
-  const size_t esz = (form == SPBTE_FORM_FP64) ? 16 : 8;
+  const size_t esz = fg_bytes(form);
   char *FG = (char *)op->d_FG;
   void *B[MAXJOBS];
   for (int q = 0; q < MAXJOBS; q++) B[q] = FG + (size_t)q * n * esz;
@@ -561,7 +573,7 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
       G1 = B[3];
     }
   }
-  LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, form != SPBTE_FORM_FP64, op->d_xs, op->s));
+  LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, fg_bytes(form) == 8, op->d_xs, op->s));
   if (ev_a) HIPCHK(hipEventRecord(ev_a, op->s));
   int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell);
   if (ev_b) HIPCHK(hipEventRecord(ev_b, op->s));
@@ -624,7 +636,7 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
   if (rc) return rc;
   const void *F = d_Fhat, *G = d_Ghat;
   const size_t n = (size_t)ncell * op->n3;
-  if (form != SPBTE_FORM_FP64) {
+  if (fg_bytes(form) == 8) {
     char *FG = (char *)op->d_FG;
     LCHK(spbte_launch_c64_to_c32(d_Fhat, FG, n, op->s));
     LCHK(spbte_launch_c64_to_c32(d_Ghat, FG + 8 * n, n, op->s));
@@ -694,7 +706,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, 
   if (rc) return rc;
   /* inputs: whatever the scratch holds; timing does not depend on values */
   const size_t n = (size_t)ncell * op->n3;
-  const size_t esz = (form == SPBTE_FORM_FP64) ? 16 : 8;
+  const size_t esz = fg_bytes(form);
   char *FG = (char *)op->d_FG;
   HIPCHK(hipMemsetAsync(FG, 0, esz * n * 4, op->s));
   const void *F0 = FG, *G0 = FG + esz * n, *F1 = FG + 2 * esz * n, *G1 = FG + 3 * esz * n;

@@ -67,6 +67,7 @@ int spbte_launch_wsum(int N, int form, const spbte_tile *tile, const spbte_wsum_
 int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, const float *rowscale,
                              hipStream_t s);
 int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s);
+int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                          double *Q, int ncell, hipStream_t s);

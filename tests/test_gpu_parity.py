@@ -237,14 +237,15 @@ def test_weights_generation_matches_fixtures(spb, orc):
 
 @pytest.mark.parametrize("N,Lv", [(16, 5.0), (24, 6.0)])
 def test_reduced_precision_forms(spb, orc, N, Lv):
-    """FP32 and FP16x2 streamed weights: <= 1e-6 relative L2 vs FP64, projection exact."""
+    """FP32 / FP16x2 streamed weights (FP64 or FP32 arithmetic): <= 1e-6 relative L2
+    vs FP64, projection exact."""
     v, eta = orc.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
     op.generate_weights_iso(1.0)
     rng = np.random.default_rng(11)
     f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(4)])
     ref = op.compute_q(f, conserve=True)
-    for form in ("fp32", "fp16x2"):
+    for form in ("fp32", "fp16x2", "fp32a"):
         for mp in (False, True):
             q = op.compute_q(f, form=form, maxpreserve=mp, conserve=True)
             r = op.compute_q(f, maxpreserve=mp, conserve=True) if mp else ref
@@ -268,5 +269,7 @@ def test_n32_single_cell_against_oracle(spb, orc):
     ref = orc.compute_q(N, v, eta, W, f, f)
     del W
     assert rel_l2(q64, ref) < 1e-12
-    for form in ("fp32", "fp16x2"):
-        assert rel_l2(op.compute_q(f[None], form=form)[0], q64) < 1e-6
+    for form in ("fp32", "fp16x2", "fp32a"):
+        err = rel_l2(op.compute_q(f[None], form=form)[0], q64)
+        print(form, err)
+        assert err < 1e-6, (form, err)
