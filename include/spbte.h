@@ -105,14 +105,16 @@ int spbte_transform(spbte_op *op, const double *d_in, double *d_out, int ncell, 
  * d_Qhat[b][r] = sum_c W[r][c] d_Ghat[b][c] d_Fhat[b][s(r,c)]. */
 int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double *d_Qhat, int ncell, int form);
 
-/* Tiles of the weighted-sum kernel: tr output rows per lane, lines z-lines of
- * output modes per workgroup, tb cells per workgroup.  set_tile pins a tile for
- * (form, maxpreserve); tr = 0 restores automatic choice.  autotune times every
- * instantiated tile on the given batch shape and pins the fastest; the chosen
- * tile is returned in *tr, *lines, *tb. */
-int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int tr, int lines, int tb);
-int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb);
-int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb);
+/* Tiles of the weighted-sum kernels.  kind 0 is the tiled kernel (tr output rows
+ * per lane, `lines` z-lines of output modes per workgroup, tb cells per
+ * workgroup); kind 1 is the streaming kernel for large batches (tr rows per
+ * lane, `lines` = waves per workgroup, tb cells per workgroup).  set_tile pins a
+ * tile for (form, maxpreserve); tr = 0 restores automatic choice.  autotune
+ * times every instantiated tile of both kinds on the given batch shape and pins
+ * the fastest for that batch size; get_tile reports the tile a batch would use. */
+int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int kind, int tr, int lines, int tb);
+int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb);
+int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb);
 
 /* Timing on the operator's stream with HIP events: runs spbte_compute_q iters
  * times and returns the mean wall time per call and the mean duration of the

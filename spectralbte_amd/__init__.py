@@ -96,9 +96,9 @@ def lib():
             "spbte_conserve_species": (I, [_vp, _vp, I]),
             "spbte_transform": (I, [_vp, _vp, _vp, I, I]),
             "spbte_qhat": (I, [_vp, _vp, _vp, _vp, I, I]),
-            "spbte_set_tile": (I, [_vp, I, I, I, I, I]),
-            "spbte_autotune": (I, [_vp, I, I, I, ip, ip, ip]),
-            "spbte_get_tile": (I, [_vp, I, I, I, ip, ip, ip]),
+            "spbte_set_tile": (I, [_vp, I, I, I, I, I, I]),
+            "spbte_autotune": (I, [_vp, I, I, I, ip, ip, ip, ip]),
+            "spbte_get_tile": (I, [_vp, I, I, I, ip, ip, ip, ip]),
             "spbte_time_compute_q": (I, [_vp, _vp, _vp, _vp, I, I, I, I, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
@@ -271,20 +271,21 @@ class Operator:
         self.sync()
         return dQ.download().view(np.complex128).reshape(shape)
 
-    def set_tile(self, form, maxpreserve, tr, lines, tb):
-        _check(lib().spbte_set_tile(self.h, FORMS[form], int(maxpreserve), tr, lines, tb))
+    def set_tile(self, form, maxpreserve, tr, lines, tb, kind=0):
+        """Pin a tile (kind 0: tiled kernel; kind 1: streaming kernel); tr=0 unpins."""
+        _check(lib().spbte_set_tile(self.h, FORMS[form], int(maxpreserve), int(kind), tr, lines, tb))
 
     def get_tile(self, ncell, form, maxpreserve=False):
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(lib().spbte_get_tile(self.h, ncell, FORMS[form], int(maxpreserve), ctypes.byref(a), ctypes.byref(b),
-                                    ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """(kind, tr, lines, tb) a batch of ncell cells would use."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().spbte_get_tile(self.h, ncell, FORMS[form], int(maxpreserve), *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, ncell, form, maxpreserve=False):
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(lib().spbte_autotune(self.h, ncell, FORMS[form], int(maxpreserve), ctypes.byref(a), ctypes.byref(b),
-                                    ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """Time every tile on this batch shape, pin the fastest; returns (kind, tr, lines, tb)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().spbte_autotune(self.h, ncell, FORMS[form], int(maxpreserve), *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def time_compute_q(self, d_f, d_g, d_Q, ncell, form="fp64", flags=0, iters=3):
         tot, ker = ctypes.c_double(), ctypes.c_double()

@@ -223,14 +223,17 @@ k_wsum(KArgs a) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const int u = t - j + 3;
-            CT pr = g[0][j].x * f[0][u].x - g[0][j].y * f[0][u].y;
-            CT pi = g[0][j].x * f[0][u].y + g[0][j].y * f[0][u].x;
+            CT pr = g[0][j].x * f[0][u].x, pi = g[0][j].x * f[0][u].y;
+            pr = fma(-g[0][j].y, f[0][u].y, pr);
+            pi = fma(g[0][j].y, f[0][u].x, pi);
             if constexpr (NP == 2) {
-              pr += g[1][j].x * f[1][u].x - g[1][j].y * f[1][u].y;
-              pi += g[1][j].x * f[1][u].y + g[1][j].y * f[1][u].x;
+              pr = fma(g[1][j].x, f[1][u].x, pr);
+              pi = fma(g[1][j].x, f[1][u].y, pi);
+              pr = fma(-g[1][j].y, f[1][u].y, pr);
+              pi = fma(g[1][j].y, f[1][u].x, pi);
             }
-            acc[b][t].x += wcur[t][j] * pr;
-            acc[b][t].y += wcur[t][j] * pi;
+            acc[b][t].x = fma(wcur[t][j], pr, acc[b][t].x);
+            acc[b][t].y = fma(wcur[t][j], pi, acc[b][t].y);
           }
       }
       if (more) {

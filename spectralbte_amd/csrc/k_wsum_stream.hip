@@ -1,0 +1,330 @@
+// k_wsum_stream.hip -- batched weighted convolution sum, streaming form.
+//
+// Same sum as k_wsum.hip (reference: src/collisions.c:125-165),
+//   Qhat_b[r] = sum_c W[r][c] ( G0_b[c] F0_b[s(r,c)] (+ G1_b[c] F1_b[s(r,c)]) ),
+//   s_d = (r_d + N/2 - c_d) mod N,
+// organised for large batches, where the kernel is bound by VALU issue rather
+// than HBM (each W element feeds TB cells, and cell groups that share an r-tile
+// share W through L2):
+//  * a lane owns TR consecutive output rows of one z-line and runs the whole
+//    sum over c itself, so there is no cross-lane reduction;
+//  * every lane of the workgroup handles the same summed mode c at the same
+//    time, so G_b[c] is wave-uniform: one broadcast LDS read per step and cell
+//    (the G x-planes are staged in LDS with the F planes);
+//  * along cz the shifted operand F_b[sy][rz + N/2 - cz] of a lane's TR rows is a
+//    window sliding by one element per cz step: it lives in registers and takes
+//    one LDS load per step, shared by TR rows (the F x-planes of the cells sit in
+//    LDS, one reload per cx);
+//  * W is read from a copy permuted for this kernel: unit (wt, c4, t, lane) holds
+//    W[row][4 c4 .. 4 c4 + 3] of row = 64 TR wt + TR lane + t, so every 16-byte
+//    wave-wide load reads 1 KiB of contiguous HBM, in consumption order.
+// Precision forms as in k_wsum.hip, FP64 arithmetic: 0 FP64 W, 1 FP32 W, 2 FP16x2 W.
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include "spbte_internal.h"
+
+namespace {
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+template <int N, int TR, int WAVES>
+struct SGeom {
+  static constexpr int RW = 64 * TR;                 // rows per wave tile
+  static constexpr int RWG = WAVES * RW;             // rows per workgroup
+  static constexpr int N2 = N * N;
+  static constexpr int P = RWG >= N2 ? RWG / N2 : 1; // x-planes of output rows per workgroup
+  static constexpr int NC4 = N * N * N / 4;          // 4-column groups per row
+  // cz steps per loop body: a multiple of TR (the circular window returns to its
+  // start) and of 8 (an even number of 4-column W chunks)
+  static constexpr int BS = (TR % 8 == 0) ? TR : ((TR % 4 == 0) ? 2 * TR : (TR % 2 == 0 ? 4 * TR : 8 * TR));
+  static_assert(N % BS == 0, "cz body must divide N");
+  static_assert(N % TR == 0, "TR must divide N");
+  static_assert((RWG % N2 == 0) || (N2 % RWG == 0), "workgroup rows must tile x-planes");
+};
+
+// Weight j (0..3) of a 16-byte unit pair of the permuted table, as a double.
+template <int FORM>
+__device__ __forceinline__ double wget(const uint4 &u, const uint4 &u2, int j) {
+  if constexpr (FORM == 0) {
+    const uint4 &q = j < 2 ? u : u2;
+    return (j & 1) ? __builtin_bit_cast(double, make_uint2(q.z, q.w)) : __builtin_bit_cast(double, make_uint2(q.x, q.y));
+  } else {
+    const unsigned d = j == 0 ? u.x : (j == 1 ? u.y : (j == 2 ? u.z : u.w));
+    if constexpr (FORM == 1) {
+      return (double)__builtin_bit_cast(float, d);
+    } else {
+      const h2v hl = __builtin_bit_cast(h2v, d);
+      return (double)hl.x + (double)hl.y;
+    }
+  }
+}
+
+template <int N, int TR, int WAVES, int TB, int NP, int FORM>
+__global__ void __launch_bounds__(64 * WAVES)
+k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
+              const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
+              double2 *__restrict__ Qh, int ncell) {
+  using Gm = SGeom<N, TR, WAVES>;
+  constexpr int N2 = Gm::N2, P = Gm::P, RWG = Gm::RWG, NC4 = Gm::NC4;
+  constexpr size_t n3 = (size_t)N * N * N;
+  constexpr int NRT = (int)(n3 / RWG);                  // r-tiles
+  constexpr int UPW = (FORM == 0) ? 2 : 1;              // 16-byte units per 4 weights
+  constexpr int BS = Gm::BS;                            // cz steps per loop body
+
+  __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N2];  // [p][b][plane][sy][sz]
+  __shared__ __attribute__((aligned(16))) double2 sG[NP * TB * N2];      // [p][b][cy][cz]
+
+  // ---- workgroup -> (r-tile, cell group), XCD-aware: consecutive workgroups
+  // on one XCD (bid mod 8) share the r-tile and so the W stream in its L2.
+  const int ncg = (ncell + TB - 1) / TB;
+  const int bid = blockIdx.x;
+  int rt, cg;
+  if ((NRT & 7) == 0) {
+    const int xcd = bid & 7, q = bid >> 3;
+    cg = q % ncg;
+    rt = (q / ncg) * 8 + xcd;
+  } else {
+    cg = bid % ncg;
+    rt = bid / ncg;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t wg_row0 = (size_t)rt * RWG;
+  const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
+  const size_t row0 = wg_row0 + (size_t)TR * threadIdx.x;
+  const int rx = (int)(row0 / N2), ry = (int)((row0 / N) % N), rz0 = (int)(row0 % N);
+  const int pl = rx - rx0;                               // this lane's plane in LDS
+  const int k0 = (rz0 + N / 2) % N;                      // window start at cz = 0 (non-negative)
+  const size_t wt = (size_t)rt * WAVES + wave;           // wave tile of the permuted table
+
+  const double2 *Fp[2] = {F0, F1};
+  const double2 *Gp[2] = {G0, G1};
+  int cells[TB];
+#pragma unroll
+  for (int b = 0; b < TB; ++b) {
+    const int c = cg * TB + b;
+    cells[b] = c < ncell ? c : ncell - 1;  // pad a ragged last group with a valid cell (not written)
+  }
+
+  double2 acc[TB][TR];
+#pragma unroll
+  for (int b = 0; b < TB; ++b)
+#pragma unroll
+    for (int t = 0; t < TR; ++t) acc[b][t] = make_double2(0.0, 0.0);
+
+  // W chunk pointer: unit index ((wt*NC4 + c4)*TR + t)*UPW*64 + h*64 + lane
+  const uint4 *Wl = Wp + (wt * NC4 * TR * UPW) * 64 + lane;
+  auto wload = [&](int c4, uint4 (&u)[TR][UPW]) {
+#pragma unroll
+    for (int t = 0; t < TR; ++t)
+#pragma unroll
+      for (int h = 0; h < UPW; ++h) u[t][h] = Wl[(((size_t)c4 * TR + t) * UPW + h) * 64];
+  };
+  // Two static register sets alternate over the 4-column chunks of W (a body
+  // holds an even number of chunks), each loaded four cz steps before use.
+  uint4 wA[TR][UPW], wB[TR][UPW];
+  wload(0, wA);
+
+  for (int cx = 0; cx < N; ++cx) {
+    __syncthreads();
+    // ---- stage the shifted F x-planes of every cell, product and lane plane
+    for (int e = threadIdx.x; e < NP * TB * P * N2; e += 64 * WAVES) {
+      const int p = e / (TB * P * N2), r1 = e - p * (TB * P * N2);
+      const int b = r1 / (P * N2), r2 = r1 - b * (P * N2);
+      const int q = r2 / N2, yz = r2 - q * N2;
+      int sx = rx0 + q + N / 2 - cx;
+      sx = sx < 0 ? sx + N : (sx > N - 1 ? sx - N : sx);
+      sF[e] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + yz];
+    }
+    for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
+      const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
+      sG[e] = Gp[p][(size_t)cells[b] * n3 + (size_t)cx * N2 + yz];
+    }
+    __syncthreads();
+
+    for (int cy = 0; cy < N; ++cy) {
+      int sy = ry + N / 2 - cy;
+      sy = sy < 0 ? sy + N : (sy > N - 1 ? sy - N : sy);
+      const double2 *line[NP][TB];
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * TB + b) * P + pl) * N + sy) * N;
+      // circular window: at step cz, element t (row rz0 + t) is buf[(t - cz) mod TR]
+      // and holds F_line[(k0 + t - cz) mod N]
+      double2 buf[NP][TB][TR];
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int b = 0; b < TB; ++b)
+#pragma unroll
+          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][((unsigned)(k0 + t)) % N];
+      const int c4base = (cx * N + cy) * (N / 4);
+      const double2 *gline = sG + (size_t)cy * N;  // G_b[cx][cy][:] of product p, cell b at + (p*TB+b)*N2
+      // One 4-step chunk: wc holds its weights, wn receives the next chunk's.
+      auto chunk = [&](uint4 (&wc)[TR][UPW], uint4 (&wn)[TR][UPW], int cz0, int c4n) __attribute__((always_inline)) {
+        if (c4n < NC4) wload(c4n, wn);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int cz = cz0 + jj;
+          const int j = cz % BS;  // position in the circular window period (cz0 % 4 == 0)
+          // rows in descending order: row 0 reads the element loaded one step
+          // earlier, last; the slot of row TR-1 is refilled right after its use
+          const unsigned idx = ((unsigned)(k0 + 2 * N - 1 - cz)) % N;
+#pragma unroll
+          for (int t = TR - 1; t >= 0; --t) {
+            const int sl = ((t - j) % TR + TR) % TR;
+            const double w = wget<FORM>(wc[t][0], wc[t][UPW - 1], jj);
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+              // complex products as explicit FMA chains: 4 ops per product, 2 for w
+              const double2 g0 = gline[(size_t)(0 * TB + b) * N2 + cz], f0 = buf[0][b][sl];
+              double pr = g0.x * f0.x, pi = g0.x * f0.y;
+              pr = fma(-g0.y, f0.y, pr);
+              pi = fma(g0.y, f0.x, pi);
+              if constexpr (NP == 2) {
+                const double2 g1 = gline[(size_t)(1 * TB + b) * N2 + cz], f1 = buf[1][b][sl];
+                pr = fma(g1.x, f1.x, pr);
+                pi = fma(g1.x, f1.y, pi);
+                pr = fma(-g1.y, f1.y, pr);
+                pi = fma(g1.y, f1.x, pi);
+              }
+              acc[b][t].x = fma(w, pr, acc[b][t].x);
+              acc[b][t].y = fma(w, pi, acc[b][t].y);
+            }
+            if (t == TR - 1) {
+              const int slnew = ((-1 - j) % TR + TR) % TR;
+#pragma unroll
+              for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int b = 0; b < TB; ++b) buf[p][b][slnew] = line[p][b][idx];
+            }
+          }
+        }
+      };
+      // The body is a whole number of window periods (TR) and of chunk pairs,
+      // so window slots and the wA/wB roles are the same at every iteration.
+#pragma unroll 1
+      for (int it = 0; it < N / BS; ++it) {
+#pragma unroll
+        for (int k = 0; k < BS / 4; k += 2) {
+          const int cz0 = it * BS + 4 * k, c4 = c4base + it * (BS / 4) + k;
+          chunk(wA, wB, cz0, c4 + 1);
+          chunk(wB, wA, cz0 + 4, c4 + 2);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int b = 0; b < TB; ++b) {
+    const int cell = cg * TB + b;
+    if (cell >= ncell) continue;
+#pragma unroll
+    for (int t = 0; t < TR; ++t) {
+      double2 s = acc[b][t];
+      if constexpr (FORM == 2) {
+        const double sc = (double)rowscale[row0 + t];
+        s.x *= sc;
+        s.y *= sc;
+      }
+      Qh[(size_t)cell * n3 + row0 + t] = s;
+    }
+  }
+}
+
+template <int N, int TR, int WAVES, int TB, int NP, int FORM>
+int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, hipStream_t s) {
+  using Gm = SGeom<N, TR, WAVES>;
+  const int ncg = (a->ncell + TB - 1) / TB;
+  const int nrt = (int)((size_t)N * N * N / Gm::RWG);
+  hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM>), dim3(nrt * ncg), dim3(64 * WAVES), 0, s,
+                     (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
+                     (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell);
+  return (int)hipGetLastError();
+}
+
+typedef int (*launch_fn)(const spbte_wsum_args *, const void *, const float *, hipStream_t);
+struct SEntry {
+  int N, tr, waves, tb, np, form;
+  launch_fn fn;
+};
+
+#define S(N, TR, WV, TB, NP, F) {N, TR, WV, TB, NP, F, &launch_s<N, TR, WV, TB, NP, F>}
+#define S3(N, TR, WV, TB, NP) S(N, TR, WV, TB, NP, 0), S(N, TR, WV, TB, NP, 1), S(N, TR, WV, TB, NP, 2)
+
+const SEntry kSTable[] = {
+    // N = 32: TR=8 (4 lanes per z-line, 2 waves per x-plane) and TR=4 (8 lanes per z-line)
+    S3(32, 8, 2, 1, 1), S3(32, 8, 2, 2, 1), S3(32, 4, 4, 2, 1), S3(32, 4, 2, 2, 1), S3(32, 4, 4, 4, 1),
+    S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S3(32, 4, 4, 2, 2), S3(32, 4, 4, 1, 2),
+    // N = 24: TR=6 (3 waves = 2 x-planes), TR=4 (9 waves = 4 x-planes)
+    S3(24, 6, 3, 1, 1), S3(24, 6, 3, 2, 1), S3(24, 6, 3, 1, 2), S3(24, 4, 9, 1, 1), S3(24, 4, 9, 2, 1),
+    S3(24, 4, 9, 1, 2),
+    // N = 16: TR=4 (1 wave = 1 x-plane)
+    S3(16, 4, 1, 1, 1), S3(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S3(16, 4, 1, 2, 2),
+};
+#undef S3
+#undef S
+
+const SEntry *sfind(int N, int form, int nprod, int tr, int waves, int tb) {
+  for (const SEntry &e : kSTable)
+    if (e.N == N && e.form == form && e.np == nprod && e.tr == tr && e.waves == waves && e.tb == tb) return &e;
+  return nullptr;
+}
+
+// Natural FP64 table -> permuted copy for tile row count TR: unit (wt, c4, t, h, lane)
+// holds W[64 TR wt + TR lane + t][4 c4 .. 4 c4 + 3] (h selects the first or second
+// pair of doubles in the FP64 form).  One thread per 4 weights.
+__global__ void __launch_bounds__(256) k_permute_w(const double *__restrict__ W, void *__restrict__ out,
+                                                   const float *__restrict__ rowscale, int form, int TR, size_t n3) {
+  const size_t nc4 = n3 / 4;
+  const size_t total = n3 * nc4;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    // i enumerates the permuted units in storage order: ((wt*nc4 + c4)*TR + t)*64 + lane
+    const size_t lane = i % 64, q1 = i / 64;
+    const size_t t = q1 % TR, q2 = q1 / TR;
+    const size_t c4 = q2 % nc4, wtile = q2 / nc4;
+    const size_t row = (wtile * 64 + lane) * TR + t;
+    const double *src = W + row * n3 + 4 * c4;
+    if (form == 0) {
+      double2 *o = (double2 *)out;
+      o[((wtile * nc4 + c4) * TR + t) * 128 + lane] = make_double2(src[0], src[1]);
+      o[((wtile * nc4 + c4) * TR + t) * 128 + 64 + lane] = make_double2(src[2], src[3]);
+    } else if (form == 1) {
+      ((float4 *)out)[i] = make_float4((float)src[0], (float)src[1], (float)src[2], (float)src[3]);
+    } else {
+      const double inv = 1.0 / (double)rowscale[row];
+      unsigned u[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double x = src[j] * inv;
+        const _Float16 hi = (_Float16)x;
+        const _Float16 lo = (_Float16)(x - (double)hi);
+        const h2v pr = {hi, lo};
+        u[j] = __builtin_bit_cast(unsigned, pr);
+      }
+      ((uint4 *)out)[i] = make_uint4(u[0], u[1], u[2], u[3]);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb) {
+  return sfind(N, form, nprod, tr, waves, tb) != nullptr;
+}
+
+// Launch with the permuted table Wp built for the same tr (spbte_launch_permute_weights).
+extern "C" int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
+                                        const float *rowscale, hipStream_t s) {
+  const SEntry *e = sfind(N, form, a->nprod, tr, waves, tb);
+  if (!e) return (int)hipErrorInvalidValue;
+  return e->fn(a, Wp, rowscale, s);
+}
+
+// rowscale (FP16x2 only) must already hold the per-row scales.
+extern "C" int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,
+                                            hipStream_t s) {
+  if (TR < 1 || n3 % (64 * (size_t)TR) || n3 % 4) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_permute_w, dim3(8192), dim3(256), 0, s, W64, dst, rowscale, form, TR, n3);
+  return (int)hipGetLastError();
+}

@@ -56,6 +56,7 @@ struct spbte_op {
   /* weight table: FP64 master copy and derived forms */
   double *dW64;
   void *dWf[NFORMS]; /* [0] aliases dW64; [1] float (forms 1, 3); [2] (hi,lo) pairs + row scales */
+  void *dWp[3][9];   /* tables permuted for the streaming kernel, per storage form and rows per lane */
   int have_w;
   /* conservation */
   int Ns;
@@ -271,6 +272,11 @@ static void free_weights(spbte_op *op) {
     if (op->dWf[f]) hipFree(op->dWf[f]);
   op->dW64 = NULL;
   for (int f = 0; f < NFORMS; f++) op->dWf[f] = NULL;
+  for (int f = 0; f < 3; f++)
+    for (int t = 0; t < 9; t++) {
+      if (op->dWp[f][t]) hipFree(op->dWp[f][t]);
+      op->dWp[f][t] = NULL;
+    }
   op->have_w = 0;
 }
 
@@ -325,8 +331,12 @@ static int ensure_scratch(spbte_op *op, int ncell) {
 /* ------------------------------------------------------------------ weights */
 static int alloc_w64(spbte_op *op) {
   if (op->dW64) {
+    HIPCHK(hipStreamSynchronize(op->s));
     for (int f = 1; f < NFORMS; f++)
       if (op->dWf[f]) { hipFree(op->dWf[f]); op->dWf[f] = NULL; }
+    for (int f = 0; f < 3; f++)
+      for (int t = 0; t < 9; t++)
+        if (op->dWp[f][t]) { hipFree(op->dWp[f][t]); op->dWp[f][t] = NULL; }
     return SPBTE_OK;
   }
   HIPCHK(hipMalloc((void **)&op->dW64, sizeof(double) * op->n3 * op->n3));
@@ -450,6 +460,22 @@ static int weights_form(spbte_op *op, int form, const void **W, const float **ro
   return SPBTE_OK;
 }
 
+/* The table permuted for the streaming kernel with tr rows per lane. */
+static int weights_permuted(spbte_op *op, int form, int tr, const void **Wp, const float **rowscale) {
+  const int st = storage_of(form);
+  if (tr < 1 || tr > 8) return fail(SPBTE_ERR_ARG, "bad rows per lane %d", tr);
+  const void *W;
+  int rc = weights_form(op, form, &W, rowscale); /* FP16x2: the row scales come from here */
+  if (rc) return rc;
+  if (!op->dWp[st][tr]) {
+    const size_t n = op->n3 * op->n3;
+    HIPCHK(hipMalloc(&op->dWp[st][tr], n * (st == SPBTE_FORM_FP64 ? 8 : 4)));
+    LCHK(spbte_launch_permute_weights(op->dW64, op->dWp[st][tr], *rowscale, st, tr, op->n3, op->s));
+  }
+  *Wp = op->dWp[st][tr];
+  return SPBTE_OK;
+}
+
 /* ------------------------------------------------------------------ tiles */
 static int bucket(int ncell) {
   int b = 0;
@@ -460,14 +486,14 @@ static int bucket(int ncell) {
 /* Default tile: enough workgroups to fill 256 CUs, cells per workgroup as the
  * batch allows (each streamed W value then feeds tb cells). */
 static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
-  static const spbte_tile c32_1[] = {{4, 4, 4}, {4, 2, 4}, {4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
-  static const spbte_tile c32_1_64[] = {{4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
-  static const spbte_tile c32_2[] = {{4, 4, 2}, {4, 2, 2}, {4, 2, 1}, {4, 1, 1}};
-  static const spbte_tile c24_1[] = {{6, 8, 4}, {6, 8, 2}, {6, 8, 1}};
-  static const spbte_tile c24_2[] = {{6, 8, 2}, {6, 8, 1}};
-  static const spbte_tile c16_1[] = {{4, 8, 4}, {4, 8, 2}, {4, 4, 2}, {4, 4, 1}};
-  static const spbte_tile c16_2[] = {{4, 8, 2}, {4, 4, 2}, {4, 4, 1}};
-  static const spbte_tile c8[] = {{2, 8, 2}};
+  static const spbte_tile c32_1[] = {{4, 4, 4, 0}, {4, 2, 4, 0}, {4, 4, 2, 0}, {4, 2, 2, 0}, {4, 2, 1, 0}, {4, 1, 1, 0}};
+  static const spbte_tile c32_1_64[] = {{4, 4, 2, 0}, {4, 2, 2, 0}, {4, 2, 1, 0}, {4, 1, 1, 0}};
+  static const spbte_tile c32_2[] = {{4, 4, 2, 0}, {4, 2, 2, 0}, {4, 2, 1, 0}, {4, 1, 1, 0}};
+  static const spbte_tile c24_1[] = {{6, 8, 4, 0}, {6, 8, 2, 0}, {6, 8, 1, 0}};
+  static const spbte_tile c24_2[] = {{6, 8, 2, 0}, {6, 8, 1, 0}};
+  static const spbte_tile c16_1[] = {{4, 8, 4, 0}, {4, 8, 2, 0}, {4, 4, 2, 0}, {4, 4, 1, 0}};
+  static const spbte_tile c16_2[] = {{4, 8, 2, 0}, {4, 4, 2, 0}, {4, 4, 1, 0}};
+  static const spbte_tile c8[] = {{2, 8, 2, 0}};
   const spbte_tile *c = NULL;
   int nc = 0;
   switch (N) {
@@ -493,45 +519,87 @@ static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   return 0;
 }
 
+/* Streaming kernel for batches large enough to fill the chip with its
+ * workgroups (one r-tile is a whole x-plane of output modes). */
+static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
+  if (form == SPBTE_FORM_FP32A) return 0;
+  static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 2, 1}, {4, 4, 1, 1}};
+  static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
+  const spbte_tile *c = NULL;
+  int nc = 0;
+  if (N == 32) { c = nprod == 2 ? s32m : s32; nc = 2; }
+  else if (N == 24) return 0; /* the 9-wave N=24 tiles cap VGPRs at 168 and spill: tiled kernel */
+  else if (N == 16) { c = nprod == 2 ? s16m : s16; nc = 2; }
+  else return 0;
+  for (int i = 0; i < nc; i++) {
+    if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
+    const long rows_wg = 64L * c[i].lines * c[i].tr, nrt = (long)N * N * N / rows_wg;
+    const long wgs = nrt * ((ncell + c[i].tb - 1) / c[i].tb);
+    if (wgs >= 2048) { *t = c[i]; return 1; }
+  }
+  return 0;
+}
+
 static int pick_tile(spbte_op *op, int ncell, int form, int nprod, spbte_tile *t) {
   if (op->pin[form][nprod - 1].tr) { *t = op->pin[form][nprod - 1]; return 1; }
   spbte_tile *tt = &op->tuned[form][nprod - 1][bucket(ncell)];
   if (tt->tr) { *t = *tt; return 1; }
+  if (default_stream_tile(op->N, form, nprod, ncell, t)) return 1;
   return default_tile(op->N, form, nprod, ncell, t);
 }
 
-int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int tr, int lines, int tb) {
+static int tile_supported(int N, int form, int nprod, const spbte_tile *t) {
+  if (t->kind == 1) return spbte_stream_tile_supported(N, form, nprod, t->tr, t->lines, t->tb);
+  return spbte_wsum_tile_supported(N, form, nprod, t);
+}
+
+int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int kind, int tr, int lines, int tb) {
   if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
   int np = maxpreserve ? 2 : 1;
-  spbte_tile t = {tr, lines, tb};
-  if (tr && !spbte_wsum_tile_supported(op->N, form, np, &t))
-    return fail(SPBTE_ERR_UNSUPPORTED, "tile (%d,%d,%d) not instantiated for N=%d form %d", tr, lines, tb, op->N, form);
+  spbte_tile t = {tr, lines, tb, kind};
+  if (tr && !tile_supported(op->N, form, np, &t))
+    return fail(SPBTE_ERR_UNSUPPORTED, "tile kind %d (%d,%d,%d) not instantiated for N=%d form %d", kind, tr, lines, tb,
+                op->N, form);
   op->pin[form][np - 1] = t;
   return SPBTE_OK;
 }
 
-int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb) {
-  spbte_tile t = {0, 0, 0};
+int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
+  spbte_tile t = {0, 0, 0, 0};
   if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
-  if (!pick_tile(op, ncell, form, maxpreserve ? 2 : 1, &t)) t.tr = t.lines = t.tb = 0;
-  *tr = t.tr; *lines = t.lines; *tb = t.tb;
+  if (!pick_tile(op, ncell, form, maxpreserve ? 2 : 1, &t)) t.tr = t.lines = t.tb = t.kind = 0;
+  *kind = t.kind; *tr = t.tr; *lines = t.lines; *tb = t.tb;
   return SPBTE_OK;
 }
 
 /* ---------------------------------------------------------- the operator */
-static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
-                    int ncell) {
-  const void *W;
+static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a) {
   const float *rs;
+  if (t && t->kind == 1) {
+    const void *Wp = NULL;
+    int rc = weights_permuted(op, form, t->tr, &Wp, &rs);
+    if (rc) return rc;
+    LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, op->s));
+    return SPBTE_OK;
+  }
+  const void *W;
   int rc = weights_form(op, form, &W, &rs);
   if (rc) return rc;
+  spbte_wsum_args b = *a;
+  b.W = W;
+  LCHK(spbte_launch_wsum_scaled(op->N, form, t, &b, rs, op->s));
+  return SPBTE_OK;
+}
+
+static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
+                    int ncell) {
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   spbte_tile t;
   int have = pick_tile(op, ncell, form, nprod, &t);
   if (!have && form != SPBTE_FORM_FP64)
     return fail(SPBTE_ERR_UNSUPPORTED, "no reduced-precision weighted-sum kernel for N=%d", op->N);
-  spbte_wsum_args a = {W, F0, G0, F1, G1, op->d_Qh, ncell, nprod};
-  LCHK(spbte_launch_wsum_scaled(op->N, form, have ? &t : NULL, &a, rs, op->s));
-  return SPBTE_OK;
+  spbte_wsum_args a = {NULL, F0, G0, F1, G1, op->d_Qh, ncell, nprod};
+  return launch_tile(op, form, have ? &t : NULL, &a);
 }
 
 /* One chunk of spbte_compute_q.  ev_a/ev_b (optional) bracket the weighted-sum
@@ -690,42 +758,44 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
   return SPBTE_OK;
 }
 
-/* Candidate tiles for the autotuner (must match k_wsum.hip's table). */
-static const spbte_tile k_cand[] = {{4, 1, 1}, {4, 1, 2}, {4, 2, 1}, {4, 2, 2}, {4, 4, 1}, {4, 4, 2}, {4, 2, 4},
-                                    {4, 4, 4}, {8, 4, 2}, {8, 8, 2}, {8, 8, 4}, {8, 4, 4}, {6, 8, 1}, {6, 8, 2},
-                                    {6, 8, 4}, {4, 8, 2}, {4, 8, 4}, {2, 8, 2}, {2, 8, 1}};
+/* Candidate tiles for the autotuner: kind 0 from k_wsum.hip's table, kind 1
+ * from k_wsum_stream.hip's (unsupported ones are skipped). */
+static const spbte_tile k_cand[] = {
+    {4, 1, 1, 0}, {4, 2, 1, 0}, {4, 2, 2, 0}, {4, 4, 2, 0}, {4, 2, 4, 0}, {4, 4, 4, 0}, {8, 4, 2, 0}, {8, 8, 2, 0},
+    {8, 4, 4, 0}, {6, 8, 1, 0}, {6, 8, 2, 0}, {6, 8, 4, 0}, {4, 8, 2, 0}, {4, 8, 4, 0}, {4, 4, 1, 0}, {2, 8, 2, 0},
+    {8, 2, 1, 1}, {8, 2, 2, 1}, {4, 4, 2, 1}, {4, 2, 2, 1}, {4, 4, 4, 1}, {4, 4, 1, 1}, {6, 3, 1, 1}, {6, 3, 2, 1},
+    {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1}, {4, 9, 2, 1}};
 
-int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, int *lines, int *tb) {
+int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   const int np = maxpreserve ? 2 : 1;
   int rc = ensure_scratch(op, ncell);
   if (rc) return rc;
-  const void *W;
-  const float *rs;
-  rc = weights_form(op, form, &W, &rs);
-  if (rc) return rc;
-  /* inputs: whatever the scratch holds; timing does not depend on values */
+  /* inputs: zeros; the timing does not depend on the values */
   const size_t n = (size_t)ncell * op->n3;
   const size_t esz = fg_bytes(form);
   char *FG = (char *)op->d_FG;
   HIPCHK(hipMemsetAsync(FG, 0, esz * n * 4, op->s));
-  const void *F0 = FG, *G0 = FG + esz * n, *F1 = FG + 2 * esz * n, *G1 = FG + 3 * esz * n;
-  spbte_tile best = {0, 0, 0};
+  spbte_wsum_args args = {NULL, FG, FG + esz * n, np == 2 ? FG + 2 * esz * n : NULL, np == 2 ? FG + 3 * esz * n : NULL,
+                          op->d_Qh, ncell, np};
+  spbte_tile best = {0, 0, 0, 0};
   float best_ms = 1e30f;
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   for (size_t i = 0; i < sizeof k_cand / sizeof k_cand[0]; i++) {
     const spbte_tile *t = &k_cand[i];
-    if (!spbte_wsum_tile_supported(op->N, form, np, t)) continue;
+    if (!tile_supported(op->N, form, np, t)) continue;
     if (t->tb > 2 * ncell) continue;
-    spbte_wsum_args args = {W, F0, G0, np == 2 ? F1 : NULL, np == 2 ? G1 : NULL, op->d_Qh, ncell, np};
-    LCHK(spbte_launch_wsum_scaled(op->N, form, t, &args, rs, op->s)); /* warm-up */
+    rc = launch_tile(op, form, t, &args); /* warm-up; builds the table layout it needs */
+    if (rc) break;
     HIPCHK(hipEventRecord(a, op->s));
     const int reps = 2;
-    for (int r = 0; r < reps; r++) LCHK(spbte_launch_wsum_scaled(op->N, form, t, &args, rs, op->s));
+    for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args);
     HIPCHK(hipEventRecord(b, op->s));
     HIPCHK(hipEventSynchronize(b));
+    if (rc) break;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, a, b));
     ms /= reps;
@@ -733,8 +803,9 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *tr, 
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
+  if (rc) return rc;
   if (!best.tr) return fail(SPBTE_ERR_UNSUPPORTED, "no tiled kernel for N=%d", op->N);
   op->tuned[form][np - 1][bucket(ncell)] = best;
-  *tr = best.tr; *lines = best.lines; *tb = best.tb;
+  *kind = best.kind; *tr = best.tr; *lines = best.lines; *tb = best.tb;
   return SPBTE_OK;
 }

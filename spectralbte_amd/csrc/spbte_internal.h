@@ -48,11 +48,13 @@ typedef struct {
   int nprod;          /* 1: ComputeQ, 2: fused maxPreserve */
 } spbte_wsum_args;
 
-/* Tile of the weighted-sum kernel (see k_wsum.hip). */
+/* Tile of a weighted-sum kernel.  kind 0: k_wsum.hip (lines = z-lines of output
+ * modes per workgroup); kind 1: k_wsum_stream.hip (lines = waves per workgroup). */
 typedef struct {
   int tr;    /* output rows (consecutive z modes) per lane */
-  int lines; /* z-lines of output modes per workgroup */
+  int lines; /* z-lines (kind 0) or waves (kind 1) per workgroup */
   int tb;    /* cells per workgroup */
+  int kind;
 } spbte_tile;
 
 /* ----- launchers (k_*.hip); all return hipError_t of the launch ----- */
@@ -69,6 +71,11 @@ int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbt
 int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s);
 int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
+int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb);
+int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
+                             const float *rowscale, hipStream_t s);
+int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,
+                                 hipStream_t s);
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                          double *Q, int ncell, hipStream_t s);
 int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t n_rows, size_t n_cols, hipStream_t s);

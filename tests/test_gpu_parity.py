@@ -89,7 +89,8 @@ def test_qhat_tiles_agree(spb, orc, N):
     op = spb.Operator(N, 5.0, v, eta)
     op.set_weights(W)
     ref = np.stack([orc.qhat(N, W, F[b], G[b]).ravel() for b in range(ncell)])
-    tiles = {16: [(4, 4, 1), (4, 4, 2), (4, 8, 2), (4, 8, 4)], 24: [(6, 8, 1), (6, 8, 2), (6, 8, 4)]}[N]
+    tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 8, 2, 0), (4, 8, 4, 0), (4, 1, 1, 1), (4, 1, 2, 1)],
+             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 8, 4, 0), (6, 3, 1, 1), (6, 3, 2, 1)]}[N]
     for t in tiles:
         op.set_tile("fp64", False, *t)
         assert rel_l2(op.qhat(F, G, "fp64"), ref) < 1e-12, t
