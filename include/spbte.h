@@ -122,6 +122,53 @@ int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
 int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
                          int iters, double *ms_per_call, double *ms_wsum);
 
+/* ---------------------------------------------------------------------------
+ * Batched time integration (replaces the per-cell loops of exec/boltz.c for one
+ * species on one rank).  A run owns the state of all cells in HBM and advances
+ * it by whole steps on the operator's stream.
+ *
+ * 0D (exec/boltz.c:193-241): ncell independent problems; each step evaluates
+ * ComputeQ_maxPreserve (or ComputeQ) + projection for all of them, then
+ * f += dt Q / Kn (order 1) or Heun (order 2).
+ *
+ * 1D (exec/boltz.c:285-354): nX interior cells with x[], dx[] of nX + 2*order
+ * entries (ghost cells included, src/mesh_setup.c).  Order 1: upwindOne
+ * (src/transportroutines.c:94-238), then per cell ComputeQ + projection and
+ * forward Euler.  Order 2: advectTwo (:477-492, minmod MUSCL), Heun collision
+ * step, advectTwo.  Walls: SPBTE_BC_DIFFUSE = diffuse reflection at T0 (left)
+ * and T1 (right) (src/boundaryConditions.c:39-85, ICChoice 3); SPBTE_BC_COPY =
+ * ghosts copy their neighbours at order 1 and the no-flux wall states at
+ * order 2; SPBTE_BC_PERIODIC = periodic ghosts (order 1, ICChoice 6).
+ *
+ * spbte_run_use_graph(run, 1) captures one step as a HIP graph on first use
+ * and replays it; the graph is captured again whenever the operator's buffers,
+ * table, species or tiles change.  The run must not outlive its operator. */
+enum { SPBTE_BC_COPY = 0, SPBTE_BC_DIFFUSE = 1, SPBTE_BC_PERIODIC = 2 };
+
+typedef struct spbte_run spbte_run;
+
+int spbte_run0d_create(spbte_run **run, spbte_op *op, int ncell, int order, double dt, double Kn, int form,
+                       int maxpreserve);
+int spbte_run1d_create(spbte_run **run, spbte_op *op, int nX, int order, const double *x, const double *dx, double dt,
+                       double Kn, int bc, double T0, double T1, int form);
+void spbte_run_destroy(spbte_run *run);
+/* Interior cells, ncell * N^3 host doubles (synchronous). */
+int spbte_run_set_state(spbte_run *run, const double *f);
+int spbte_run_get_state(spbte_run *run, double *f);
+/* Device pointer to the interior cells of the state. */
+int spbte_run_state(spbte_run *run, double **d_f);
+int spbte_run_use_graph(spbte_run *run, int on);
+/* Queue nsteps steps (asynchronous; spbte_op_synchronize waits). */
+int spbte_run_step(spbte_run *run, int nsteps);
+/* Per cell rho, u_x, u_y, u_z, T (src/momentRoutines.c:58-183): 5 * ncell host doubles. */
+int spbte_run_moments(spbte_run *run, double *out);
+
+/* One transport stage alone, synchronous: order 1 = upwindOne, order 2 =
+ * advectTwo, from d_f to d_fc, both (nX + 2*order) * N^3 device doubles with
+ * ghost cells; d_f's ghost cells are overwritten as in the reference. */
+int spbte_advect(spbte_op *op, double *d_f, double *d_fc, int nX, int order, const double *x, const double *dx,
+                 double dt, int bc, double T0, double T1);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,16 @@ def lib():
             "spbte_autotune": (I, [_vp, I, I, I, ip, ip, ip, ip]),
             "spbte_get_tile": (I, [_vp, I, I, I, ip, ip, ip, ip]),
             "spbte_time_compute_q": (I, [_vp, _vp, _vp, _vp, I, I, I, I, _dp, _dp]),
+            "spbte_run0d_create": (I, [ctypes.POINTER(_vp), _vp, I, I, D, D, I, I]),
+            "spbte_run1d_create": (I, [ctypes.POINTER(_vp), _vp, I, I, _dp, _dp, D, D, I, D, D, I]),
+            "spbte_run_destroy": (None, [_vp]),
+            "spbte_run_set_state": (I, [_vp, _dp]),
+            "spbte_run_get_state": (I, [_vp, _dp]),
+            "spbte_run_state": (I, [_vp, ctypes.POINTER(_vp)]),
+            "spbte_run_use_graph": (I, [_vp, I]),
+            "spbte_run_step": (I, [_vp, I]),
+            "spbte_run_moments": (I, [_vp, _dp]),
+            "spbte_advect": (I, [_vp, _vp, _vp, I, I, _dp, _dp, D, I, D, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -292,6 +302,95 @@ class Operator:
         _check(lib().spbte_time_compute_q(self.h, d_f, d_g, d_Q, int(ncell), FORMS[form], int(flags), int(iters),
                                           ctypes.byref(tot), ctypes.byref(ker)))
         return tot.value, ker.value
+
+
+# ------------------------------------------------------ batched time stepping
+BC_COPY, BC_DIFFUSE, BC_PERIODIC = 0, 1, 2
+
+
+class Run:
+    """spbte_run: the state of a batch of cells in HBM, advanced by whole steps
+    (exec/boltz.c:193-241 for 0D, :285-354 for 1D).  Create with ``Run.ensemble``
+    or ``Run.mesh``."""
+
+    def __init__(self, op, handle, ncell):
+        self.op, self.h, self.ncell = op, handle, ncell
+        self.n3 = op.N ** 3
+
+    @classmethod
+    def ensemble(cls, op, f0, order=2, dt=0.01, Kn=1.0, form="fp64", maxpreserve=True):
+        f0 = _c(f0).reshape(-1, op.N ** 3)
+        h = _vp()
+        _check(lib().spbte_run0d_create(ctypes.byref(h), op.h, f0.shape[0], order, dt, Kn, FORMS[form],
+                                        int(maxpreserve)))
+        r = cls(op, h, f0.shape[0])
+        r.set_state(f0)
+        return r
+
+    @classmethod
+    def mesh(cls, op, f0, x, dx, order=1, dt=0.01, Kn=1.0, bc=BC_DIFFUSE, T0=1.0, T1=2.0, form="fp64"):
+        """f0: (nX, N^3) interior cells; x, dx: nX + 2*order entries with ghosts."""
+        f0 = _c(f0).reshape(-1, op.N ** 3)
+        x, dx = _c(x), _c(dx)
+        assert x.size == f0.shape[0] + 2 * order and dx.size == x.size
+        h = _vp()
+        _check(lib().spbte_run1d_create(ctypes.byref(h), op.h, f0.shape[0], order, _p(x), _p(dx), dt, Kn, bc, T0, T1,
+                                        FORMS[form]))
+        r = cls(op, h, f0.shape[0])
+        r.set_state(f0)
+        return r
+
+    def set_state(self, f):
+        f = _c(f)
+        assert f.size == self.ncell * self.n3
+        _check(lib().spbte_run_set_state(self.h, _p(f)))
+
+    def state(self):
+        out = np.empty((self.ncell, self.n3))
+        _check(lib().spbte_run_get_state(self.h, _p(out)))
+        return out
+
+    def state_ptr(self):
+        p = _vp()
+        _check(lib().spbte_run_state(self.h, ctypes.byref(p)))
+        return p
+
+    def use_graph(self, on=True):
+        _check(lib().spbte_run_use_graph(self.h, int(on)))
+
+    def step(self, nsteps=1, sync=True):
+        _check(lib().spbte_run_step(self.h, int(nsteps)))
+        if sync:
+            self.op.sync()
+
+    def moments(self):
+        """(ncell, 5): rho, u_x, u_y, u_z, T per cell."""
+        out = np.empty((self.ncell, 5))
+        _check(lib().spbte_run_moments(self.h, _p(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().spbte_run_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def advect(op, f, x, dx, order, dt, bc, T0=1.0, T1=2.0):
+    """One transport stage on the GPU (upwindOne or advectTwo): f is
+    (nX + 2*order, N^3) with ghost cells; returns (f with its ghosts as the
+    stage left them, fc)."""
+    f = _c(f)
+    nX = f.shape[0] - 2 * order
+    d_f = DeviceArray.from_host(f)
+    d_fc = DeviceArray(f.shape, np.zeros_like(f))
+    _check(lib().spbte_advect(op.h, d_f.ptr, d_fc.ptr, nX, order, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1))
+    return d_f.download(), d_fc.download()
 
 
 # ------------------------------------------------ reference interface (numpy)

@@ -29,6 +29,15 @@ static int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+/* The same, for the other host modules of the library. */
+int spbte_fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
 #define HIPCHK(expr)                                                                                  \
   do {                                                                                                \
     hipError_t e_ = (expr);                                                                           \
@@ -73,6 +82,9 @@ struct spbte_op {
   spbte_tile pin[NFORMS][2];
   /* autotuned tiles per (form, nprod-1, ncell bucket) */
   spbte_tile tuned[NFORMS][2][16];
+  /* bumped whenever a device buffer, the table, the species or a tile changes:
+   * launch sequences captured in a graph are valid for one generation */
+  unsigned gen;
 };
 
 int spbte_device_count(int *n) {
@@ -111,6 +123,8 @@ int spbte_op_synchronize(spbte_op *op) {
 
 hipStream_t spbte_op_stream(spbte_op *op) { return op->s; }
 const spbte_grid_dev *spbte_op_grid(spbte_op *op) { return &op->g; }
+unsigned spbte_op_generation(spbte_op *op) { return op->gen; }
+int spbte_op_n(spbte_op *op) { return op->N; }
 
 /* ------------------------------------------------------ transform matrices */
 
@@ -212,6 +226,7 @@ int spbte_op_set_species(spbte_op *op, int Ns, const double *masses) {
   if (spbte_host_lu_factor(Ns + 4, op->LU, op->pivot) != 0) return fail(SPBTE_ERR_SINGULAR, "C C^T is singular");
   op->g.mass = op->masses[0];
   op->g.kb = (op->masses[0] == 1.0) ? 1.0 : 1.380658e-23;
+  op->gen++;
   return SPBTE_OK;
 }
 
@@ -325,11 +340,13 @@ static int ensure_scratch(spbte_op *op, int ncell) {
   HIPCHK(hipMalloc(&op->d_FG, 16 * n * MAXJOBS));
   HIPCHK(hipMalloc(&op->d_Qh, 16 * n));
   op->cap = ncell;
+  op->gen++;
   return SPBTE_OK;
 }
 
 /* ------------------------------------------------------------------ weights */
 static int alloc_w64(spbte_op *op) {
+  op->gen++;
   if (op->dW64) {
     HIPCHK(hipStreamSynchronize(op->s));
     for (int f = 1; f < NFORMS; f++)
@@ -453,6 +470,7 @@ static int weights_form(spbte_op *op, int form, const void **W, const float **ro
   if (!op->dWf[st]) {
     size_t bytes = (st == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
     HIPCHK(hipMalloc(&op->dWf[st], bytes));
+    op->gen++;
     LCHK(spbte_launch_convert_weights(op->dW64, op->dWf[st], st, op->n3, op->n3, op->s));
   }
   *W = op->dWf[st];
@@ -470,6 +488,7 @@ static int weights_permuted(spbte_op *op, int form, int tr, const void **Wp, con
   if (!op->dWp[st][tr]) {
     const size_t n = op->n3 * op->n3;
     HIPCHK(hipMalloc(&op->dWp[st][tr], n * (st == SPBTE_FORM_FP64 ? 8 : 4)));
+    op->gen++;
     LCHK(spbte_launch_permute_weights(op->dW64, op->dWp[st][tr], *rowscale, st, tr, op->n3, op->s));
   }
   *Wp = op->dWp[st][tr];
@@ -561,6 +580,7 @@ int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int kind, int tr, in
     return fail(SPBTE_ERR_UNSUPPORTED, "tile kind %d (%d,%d,%d) not instantiated for N=%d form %d", kind, tr, lines, tb,
                 op->N, form);
   op->pin[form][np - 1] = t;
+  op->gen++;
   return SPBTE_OK;
 }
 
@@ -806,6 +826,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
   if (rc) return rc;
   if (!best.tr) return fail(SPBTE_ERR_UNSUPPORTED, "no tiled kernel for N=%d", op->N);
   op->tuned[form][np - 1][bucket(ncell)] = best;
+  op->gen++;
   *kind = best.kind; *tr = best.tr; *lines = best.lines; *tb = best.tb;
   return SPBTE_OK;
 }

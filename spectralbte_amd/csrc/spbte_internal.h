@@ -82,6 +82,19 @@ int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t 
 int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
                              double *W, hipStream_t s);
 
+/* transport and time-integration kernels (k_transport.hip) */
+int spbte_launch_upwind1(int N, int nX, const double *f, double *fc, const double *v, const double *dx, double dt,
+                         hipStream_t s);
+int spbte_launch_ghosts(int N, int nX, int mode, double *f, hipStream_t s);
+int spbte_launch_diffuse(const spbte_grid_dev *g, int nwall, const double *const *src, double *const *dst,
+                         const double *TW, const int *right, hipStream_t s);
+int spbte_launch_walls2(int N, int nX, int diffuse, const double *f, double *fl, double *fr, const double *x,
+                        const double *dx, hipStream_t s);
+int spbte_launch_upwind2(int N, int nX, const double *f, double *fc, const double *fl, const double *fr,
+                         const double *v, const double *x, const double *dx, double dt, hipStream_t s);
+int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const double *y, double c, const double *z,
+                         double *out, hipStream_t s);
+
 /* host helpers shared by the reference-compatible modules (spbte_core.c) */
 int spbte_host_lu_factor(int n, double *A, int *pivot);
 void spbte_host_lu_solve(int n, const double *LU, const int *pivot, double *b);
@@ -90,6 +103,9 @@ struct spbte_op;
 hipStream_t spbte_op_stream(struct spbte_op *op);
 const spbte_grid_dev *spbte_op_grid(struct spbte_op *op);
 int spbte_op_get_weights_rows(struct spbte_op *op, double **rows);
+unsigned spbte_op_generation(struct spbte_op *op);
+int spbte_op_n(struct spbte_op *op);
+int spbte_fail(int code, const char *fmt, ...);
 
 #ifdef __cplusplus
 }

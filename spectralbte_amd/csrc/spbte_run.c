@@ -1,0 +1,335 @@
+/* spbte_run.c -- batched time integration on one GPU (C99).
+ *
+ * Replaces the per-cell driver loops of exec/boltz.c for one species on one
+ * rank: the 0D loop (:193-241: ComputeQ_maxPreserve + conserveAllMoments, then
+ * forward Euler or Heun) and the 1D loop (:285-354: transport, then per-cell
+ * ComputeQ + conserveAllMoments with Euler or Heun, then for order 2 a second
+ * transport stage).  All cells are one batch: a step is a fixed sequence of
+ * launches on the operator's stream (transport kernels, one batched collision
+ * evaluation per collision stage, elementwise updates), and it can be captured
+ * once as a HIP graph and replayed.  The state never leaves HBM between steps.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "spbte.h"
+#include "spbte_internal.h"
+
+#define HIPCHK(expr)                                                                                        \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess) return spbte_fail(SPBTE_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                            __FILE__, __LINE__);                                            \
+  } while (0)
+
+#define CHK(expr)               \
+  do {                          \
+    int r_ = (expr);            \
+    if (r_ != 0) return r_;     \
+  } while (0)
+
+#define LCHK(expr)                                                                                             \
+  do {                                                                                                         \
+    int e_ = (expr);                                                                                           \
+    if (e_ != 0) return spbte_fail(SPBTE_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString((hipError_t)e_), \
+                                   __FILE__, __LINE__);                                                        \
+  } while (0)
+
+struct spbte_run {
+  spbte_op *op;
+  int dim;      /* 0: ensemble of 0D problems; 1: 1D mesh */
+  int ncell;    /* problems (0D) or interior cells nX (1D) */
+  int order;    /* 1: Euler (+ upwindOne); 2: Heun (+ advectTwo) */
+  int form, flags;
+  double dt, Kn;
+  int bc;       /* 1D walls: SPBTE_BC_* */
+  double T0, T1;
+  size_t n3;
+  int nghost;   /* ghost cells per side (1D: order; 0D: 0) */
+  double *d_x, *d_dx;            /* 1D: nX + 2 order */
+  double *d_f, *d_fc, *d_tmp;    /* state (with ghosts), transported state, advectTwo stage */
+  double *d_f1, *d_Q;            /* Heun stage, collision operator */
+  double *d_fl, *d_fr;           /* order-2 wall states */
+  double *d_mom;
+  int use_graph;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  unsigned gen;
+};
+
+static int dev_alloc(double **p, size_t n) {
+  HIPCHK(hipMalloc((void **)p, sizeof(double) * (n ? n : 1)));
+  HIPCHK(hipMemset(*p, 0, sizeof(double) * (n ? n : 1)));
+  return SPBTE_OK;
+}
+
+static void drop_graph(spbte_run *r) {
+  if (r->exec) hipGraphExecDestroy(r->exec);
+  if (r->graph) hipGraphDestroy(r->graph);
+  r->exec = NULL;
+  r->graph = NULL;
+}
+
+void spbte_run_destroy(spbte_run *r) {
+  if (!r) return;
+  hipStreamSynchronize(spbte_op_stream(r->op));
+  drop_graph(r);
+  double *bufs[] = {r->d_x, r->d_dx, r->d_f, r->d_fc, r->d_tmp, r->d_f1, r->d_Q, r->d_fl, r->d_fr, r->d_mom};
+  for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; i++)
+    if (bufs[i]) hipFree(bufs[i]);
+  free(r);
+}
+
+static int run_new(spbte_run **out, spbte_op *op, int dim, int ncell, int order, double dt, double Kn, int form) {
+  if (!out || !op) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  if (ncell < 1) return spbte_fail(SPBTE_ERR_ARG, "ncell=%d must be positive", ncell);
+  if (order != 1 && order != 2) return spbte_fail(SPBTE_ERR_ARG, "order=%d must be 1 or 2", order);
+  if (form < 0 || form > SPBTE_FORM_FP32A) return spbte_fail(SPBTE_ERR_ARG, "bad form %d", form);
+  if (!(Kn > 0.0)) return spbte_fail(SPBTE_ERR_ARG, "Kn must be positive");
+  spbte_run *r = calloc(1, sizeof *r);
+  if (!r) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+  const int N = spbte_op_n(op);
+  r->op = op;
+  r->dim = dim;
+  r->ncell = ncell;
+  r->order = order;
+  r->dt = dt;
+  r->Kn = Kn;
+  r->form = form;
+  r->n3 = (size_t)N * N * N;
+  *out = r;
+  return SPBTE_OK;
+}
+
+int spbte_run0d_create(spbte_run **out, spbte_op *op, int ncell, int order, double dt, double Kn, int form,
+                       int maxpreserve) {
+  CHK(run_new(out, op, 0, ncell, order, dt, Kn, form));
+  spbte_run *r = *out;
+  r->flags = SPBTE_SELF | SPBTE_CONSERVE | (maxpreserve ? SPBTE_MAXPRESERVE : 0);
+  int rc = dev_alloc(&r->d_f, ncell * r->n3);
+  if (!rc) rc = dev_alloc(&r->d_Q, ncell * r->n3);
+  if (!rc && order == 2) rc = dev_alloc(&r->d_f1, ncell * r->n3);
+  if (!rc) rc = dev_alloc(&r->d_mom, (size_t)ncell * SPBTE_MOM_STRIDE);
+  if (rc) { spbte_run_destroy(r); *out = NULL; }
+  return rc;
+}
+
+int spbte_run1d_create(spbte_run **out, spbte_op *op, int nX, int order, const double *x, const double *dx, double dt,
+                       double Kn, int bc, double T0, double T1, int form) {
+  if (!x || !dx) return spbte_fail(SPBTE_ERR_ARG, "x and dx are required");
+  if (bc < SPBTE_BC_COPY || bc > SPBTE_BC_PERIODIC) return spbte_fail(SPBTE_ERR_ARG, "bad boundary condition %d", bc);
+  if (order == 2 && bc == SPBTE_BC_PERIODIC)
+    return spbte_fail(SPBTE_ERR_UNSUPPORTED, "periodic ghosts exist for order 1 only (src/transportroutines.c:156-172)");
+  if (order == 2 && nX < 2) return spbte_fail(SPBTE_ERR_ARG, "order 2 needs nX >= 2");
+  CHK(run_new(out, op, 1, nX, order, dt, Kn, form));
+  spbte_run *r = *out;
+  r->flags = SPBTE_SELF | SPBTE_CONSERVE;
+  r->bc = bc;
+  r->T0 = T0;
+  r->T1 = T1;
+  r->nghost = order;
+  const size_t nc = (size_t)nX + 2 * order;
+  int rc = dev_alloc(&r->d_x, nc);
+  if (!rc) rc = dev_alloc(&r->d_dx, nc);
+  if (!rc) rc = dev_alloc(&r->d_f, nc * r->n3);
+  if (!rc) rc = dev_alloc(&r->d_fc, nc * r->n3);
+  if (!rc) rc = dev_alloc(&r->d_Q, (size_t)nX * r->n3);
+  if (!rc && order == 2) rc = dev_alloc(&r->d_tmp, nc * r->n3);
+  if (!rc && order == 2) rc = dev_alloc(&r->d_f1, (size_t)nX * r->n3);
+  if (!rc && order == 2) rc = dev_alloc(&r->d_fl, r->n3);
+  if (!rc && order == 2) rc = dev_alloc(&r->d_fr, r->n3);
+  if (!rc) rc = dev_alloc(&r->d_mom, (size_t)nX * SPBTE_MOM_STRIDE);
+  if (!rc && hipMemcpy(r->d_x, x, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess)
+    rc = spbte_fail(SPBTE_ERR_HIP, "copy of x failed");
+  if (!rc && hipMemcpy(r->d_dx, dx, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess)
+    rc = spbte_fail(SPBTE_ERR_HIP, "copy of dx failed");
+  if (rc) { spbte_run_destroy(r); *out = NULL; }
+  return rc;
+}
+
+static double *interior(const spbte_run *r, double *buf) { return buf + (size_t)r->nghost * r->n3; }
+
+int spbte_run_set_state(spbte_run *r, const double *f) {
+  if (!r || !f) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(spbte_op_stream(r->op)));
+  HIPCHK(hipMemcpy(interior(r, r->d_f), f, sizeof(double) * r->ncell * r->n3, hipMemcpyHostToDevice));
+  return SPBTE_OK;
+}
+
+int spbte_run_get_state(spbte_run *r, double *f) {
+  if (!r || !f) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(spbte_op_stream(r->op)));
+  HIPCHK(hipMemcpy(f, interior(r, r->d_f), sizeof(double) * r->ncell * r->n3, hipMemcpyDeviceToHost));
+  return SPBTE_OK;
+}
+
+int spbte_run_state(spbte_run *r, double **d_f) {
+  if (!r || !d_f) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  *d_f = interior(r, r->d_f);
+  return SPBTE_OK;
+}
+
+int spbte_run_moments(spbte_run *r, double *out) {
+  if (!r || !out) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  hipStream_t s = spbte_op_stream(r->op);
+  LCHK(spbte_launch_moments(spbte_op_grid(r->op), interior(r, r->d_f), r->d_mom, r->ncell, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double *m = malloc(sizeof(double) * r->ncell * SPBTE_MOM_STRIDE);
+  if (!m) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+  hipError_t e = hipMemcpy(m, r->d_mom, sizeof(double) * r->ncell * SPBTE_MOM_STRIDE, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (int c = 0; c < r->ncell; c++)
+      for (int q = 0; q < 5; q++) out[5 * c + q] = m[SPBTE_MOM_STRIDE * c + q];
+  free(m);
+  if (e != hipSuccess) return spbte_fail(SPBTE_ERR_HIP, "moments copy: %s", hipGetErrorString(e));
+  return SPBTE_OK;
+}
+
+/* ---------------------------------------------------------------- transport */
+/* One upwindTwo stage src -> dst (src/transportroutines.c:241-470): ghost
+ * extrapolation in src, wall states, diffuse walls, MUSCL half-step. */
+static int upwind_two(spbte_run *r, const double *d_x, const double *d_dx, double *src, double *dst) {
+  const spbte_grid_dev *g = spbte_op_grid(r->op);
+  hipStream_t s = spbte_op_stream(r->op);
+  const int N = g->N, nX = r->ncell, diffuse = r->bc == SPBTE_BC_DIFFUSE;
+  LCHK(spbte_launch_ghosts(N, nX, 3, src, s));
+  LCHK(spbte_launch_walls2(N, nX, diffuse, src, r->d_fl, r->d_fr, d_x, d_dx, s));
+  if (diffuse) {
+    const double *ws[2] = {r->d_fl, r->d_fr};
+    double *wd[2] = {r->d_fl, r->d_fr};
+    const double TW[2] = {r->T0, r->T1};
+    const int right[2] = {0, 1};
+    LCHK(spbte_launch_diffuse(g, 2, ws, wd, TW, right, s));
+  }
+  LCHK(spbte_launch_upwind2(N, nX, src, dst, r->d_fl, r->d_fr, (const double *)g->d_v, d_x, d_dx, r->dt, s));
+  return SPBTE_OK;
+}
+
+/* Transport src -> dst over the interior cells: upwindOne (order 1,
+ * src/transportroutines.c:94-238) or advectTwo (order 2, :477-492). */
+static int advect(spbte_run *r, double *src, double *dst) {
+  const spbte_grid_dev *g = spbte_op_grid(r->op);
+  hipStream_t s = spbte_op_stream(r->op);
+  const int N = g->N, nX = r->ncell;
+  if (r->order == 1) {
+    if (r->bc == SPBTE_BC_DIFFUSE) {
+      const double *ws[2] = {src + r->n3, src + (size_t)nX * r->n3};
+      double *wd[2] = {src, src + (size_t)(nX + 1) * r->n3};
+      const double TW[2] = {r->T0, r->T1};
+      const int right[2] = {0, 1};
+      LCHK(spbte_launch_diffuse(g, 2, ws, wd, TW, right, s));
+    } else {
+      LCHK(spbte_launch_ghosts(N, nX, r->bc == SPBTE_BC_PERIODIC ? 2 : 0, src, s));
+    }
+    LCHK(spbte_launch_upwind1(N, nX, src, dst, (const double *)g->d_v, r->d_dx, r->dt, s));
+    return SPBTE_OK;
+  }
+  CHK(upwind_two(r, r->d_x, r->d_dx, src, r->d_tmp));
+  CHK(upwind_two(r, r->d_x, r->d_dx, r->d_tmp, dst));
+  const size_t n = (size_t)nX * r->n3;
+  LCHK(spbte_launch_lincomb(n, 0.5, interior(r, src), 0.5, interior(r, dst), 0.0, NULL, interior(r, dst), s));
+  return SPBTE_OK;
+}
+
+/* ------------------------------------------------------------------- a step */
+static int collide(spbte_run *r, const double *d_in) {
+  return spbte_compute_q(r->op, d_in, NULL, r->d_Q, r->ncell, r->form, r->flags);
+}
+
+static int enqueue_step(spbte_run *r) {
+  hipStream_t s = spbte_op_stream(r->op);
+  const size_t n = (size_t)r->ncell * r->n3;
+  const double h = r->dt / r->Kn;
+  double *f = interior(r, r->d_f);
+  if (r->dim == 0) {
+    /* exec/boltz.c:193-241 */
+    CHK(collide(r, f));
+    if (r->order == 1) {
+      LCHK(spbte_launch_lincomb(n, 1.0, f, h, r->d_Q, 0.0, NULL, f, s));
+    } else {
+      LCHK(spbte_launch_lincomb(n, 1.0, f, h, r->d_Q, 0.0, NULL, r->d_f1, s));
+      CHK(collide(r, r->d_f1));
+      LCHK(spbte_launch_lincomb(n, 0.5, f, 0.5, r->d_f1, 0.5 * h, r->d_Q, f, s));
+    }
+    return SPBTE_OK;
+  }
+  /* exec/boltz.c:285-354 */
+  double *fc = interior(r, r->d_fc);
+  CHK(advect(r, r->d_f, r->d_fc));
+  CHK(collide(r, fc));
+  if (r->order == 1) {
+    LCHK(spbte_launch_lincomb(n, 1.0, fc, h, r->d_Q, 0.0, NULL, f, s));
+    return SPBTE_OK;
+  }
+  LCHK(spbte_launch_lincomb(n, 1.0, fc, h, r->d_Q, 0.0, NULL, r->d_f1, s));
+  CHK(collide(r, r->d_f1));
+  LCHK(spbte_launch_lincomb(n, 0.5, fc, 0.5, r->d_f1, 0.5 * h, r->d_Q, fc, s));
+  CHK(advect(r, r->d_fc, r->d_f));
+  return SPBTE_OK;
+}
+
+/* Capture one step as a graph.  Everything a step allocates on first use
+ * (operator scratch, derived weight tables) is allocated by an evaluation of
+ * the operator before the capture starts, so the captured sequence contains
+ * kernel launches only. */
+static int capture(spbte_run *r) {
+  hipStream_t s = spbte_op_stream(r->op);
+  drop_graph(r);
+  CHK(collide(r, interior(r, r->d_f)));
+  HIPCHK(hipStreamSynchronize(s));
+  const unsigned gen = spbte_op_generation(r->op);
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(r);
+  hipGraph_t graph = NULL;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return spbte_fail(SPBTE_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+  r->graph = graph;
+  HIPCHK(hipGraphInstantiate(&r->exec, r->graph, NULL, NULL, 0));
+  if (spbte_op_generation(r->op) != gen) {
+    drop_graph(r);
+    return spbte_fail(SPBTE_ERR_HIP, "operator state changed during capture");
+  }
+  r->gen = gen;
+  return SPBTE_OK;
+}
+
+int spbte_run_use_graph(spbte_run *r, int on) {
+  if (!r) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  r->use_graph = on != 0;
+  if (!r->use_graph) {
+    HIPCHK(hipStreamSynchronize(spbte_op_stream(r->op)));
+    drop_graph(r);
+  }
+  return SPBTE_OK;
+}
+
+int spbte_run_step(spbte_run *r, int nsteps) {
+  if (!r || nsteps < 0) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  hipStream_t s = spbte_op_stream(r->op);
+  for (int t = 0; t < nsteps; t++) {
+    if (r->use_graph) {
+      if (!r->exec || r->gen != spbte_op_generation(r->op)) CHK(capture(r));
+      HIPCHK(hipGraphLaunch(r->exec, s));
+    } else {
+      CHK(enqueue_step(r));
+    }
+  }
+  return SPBTE_OK;
+}
+
+/* ------------------------------------------------------ standalone transport */
+int spbte_advect(spbte_op *op, double *d_f, double *d_fc, int nX, int order, const double *x, const double *dx,
+                 double dt, int bc, double T0, double T1) {
+  if (!op || !d_f || !d_fc || !x || !dx || nX < 1) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  spbte_run *r = NULL;
+  CHK(spbte_run1d_create(&r, op, nX, order, x, dx, dt, 1.0, bc, T0, T1, SPBTE_FORM_FP64));
+  int rc = advect(r, d_f, d_fc);
+  if (!rc && hipStreamSynchronize(spbte_op_stream(op)) != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "advect failed");
+  spbte_run_destroy(r);
+  return rc;
+}

@@ -1,0 +1,227 @@
+"""GPU parity of the batched time integration (spbte_run*) and the on-device
+transport (spbte_advect) against the CPU oracle and the reference fixtures.
+
+Tolerances: transport stages 1e-13 of max|f| (elementwise FP64, same operation
+order); whole runs 1e-10 relative on the moments against the oracle's run (FP64
+round-off accumulated over many steps) and the fixtures' own print precision
+(5e-7) against the reference output; graph replays are bit-identical to eager
+steps (same kernels, no atomics)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _shock_cells(orc, v, nX):
+    """Mach-1.2-like step between two Maxwellians, blended over a few cells
+    (1D convention of src/initializer.c:366,410)."""
+    up = orc.maxwellian_1d(v, 1.0, 1.2972, 1.0).ravel()
+    dn = orc.maxwellian_1d(v, 1.297, 1.0, 1.195).ravel()
+    s = 0.5 * (1 + np.tanh((np.arange(nX) + 0.5 - nX / 2) / 1.5))
+    return (1 - s)[:, None] * up[None, :] + s[:, None] * dn[None, :]
+
+
+def _ghosted(f, order, rng):
+    nX = f.shape[0]
+    out = np.empty((nX + 2 * order, f.shape[1]))
+    out[order:order + nX] = f
+    out[:order] = rng.uniform(0, 1, (order, f.shape[1]))       # ghosts are overwritten or unread
+    out[order + nX:] = rng.uniform(0, 1, (order, f.shape[1]))
+    return out
+
+
+@pytest.mark.parametrize("bc", [0, 1, 2])
+def test_advect_order1(spb, orc, bc):
+    """upwindOne (src/transportroutines.c:94-238): copy / diffuse / periodic ghosts."""
+    N, Lv, nX, dt = 8, 5.0, 7, 0.004
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    rng = np.random.default_rng(10 + bc)
+    f = _ghosted(_shock_cells(orc, v, nX) * rng.uniform(0.9, 1.1, (nX, 1)), 1, rng)
+    x, dx = orc.uniform_mesh(nX, 1.0, 1)
+    gf, gfc = spb.advect(op, f, x, dx, 1, dt, bc)
+    of, ofc = orc.upwind_one(N, nX, v, dx, dt, bc, f)
+    scale = np.abs(f[1:nX + 1]).max()
+    assert np.abs(gfc[1:nX + 1] - ofc[1:nX + 1]).max() < 1e-13 * scale
+    # the ghost halves the stage reads (incoming velocities)
+    inc_l = (slice(None),) if bc != 1 else (slice(N ** 3 // 2, None),)
+    inc_r = (slice(None),) if bc != 1 else (slice(0, N ** 3 // 2),)
+    assert np.abs(gf[0][inc_l] - of[0][inc_l]).max() < 1e-13 * scale
+    assert np.abs(gf[nX + 1][inc_r] - of[nX + 1][inc_r]).max() < 1e-13 * scale
+    op.close()
+
+
+def test_advect_order2_uniform_state_is_fixed(spb, orc):
+    """With no-flux walls a spatially uniform state is a fixed point of advectTwo
+    (all slopes vanish and the wall states equal the wall cells)."""
+    N, Lv, nX = 8, 5.0, 6
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    f = np.tile(orc.maxwellian_1d(v, 1.1, 0.3, 0.9).ravel(), (nX + 4, 1))
+    x, dx = orc.uniform_mesh(nX, 1.0, 2)
+    _, fc = spb.advect(op, f, x, dx, 2, 0.003, 0)
+    assert np.array_equal(fc[2:nX + 2], f[2:nX + 2])
+    op.close()
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_advect_order2(spb, orc, bc):
+    """advectTwo (src/transportroutines.c:477-492): two minmod MUSCL stages and
+    their average; no-flux or diffuse walls."""
+    N, Lv, nX, dt = 8, 5.0, 9, 0.004
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    rng = np.random.default_rng(20 + bc)
+    f = _ghosted(_shock_cells(orc, v, nX) * rng.uniform(0.8, 1.2, (nX, 1)), 2, rng)
+    x, dx = orc.uniform_mesh(nX, 1.0, 2)
+    gf, gfc = spb.advect(op, f, x, dx, 2, dt, bc)
+    of, ofc = orc.advect_two(N, nX, v, x, dx, dt, bc, f)
+    scale = np.abs(f[2:nX + 2]).max()
+    assert np.abs(gfc[2:nX + 2] - ofc[2:nX + 2]).max() < 1e-13 * scale
+    assert np.abs(gf[1] - of[1]).max() < 1e-13 * scale and np.abs(gf[nX + 2] - of[nX + 2]).max() < 1e-13 * scale
+    op.close()
+
+
+def test_run0d_bkw8_fixture(spb, orc):
+    """tests/BKW8 of the reference as one batched run: 100 Heun steps of
+    ComputeQ_maxPreserve + projection entirely on the GPU."""
+    N, Lv, dt = 8, 5.0, 0.01
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_load(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), N)
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_weights(W)
+    f0 = orc.bkw_initial(N, v).ravel()
+    run = spb.Run.ensemble(op, np.stack([f0, f0]), order=2, dt=dt)   # two identical problems
+    rows = [orc.moments_row_0d(N, v, f0, 0.0)]
+    for t in range(100):
+        run.step(1)
+        rows.append(orc.moments_row_0d(N, v, run.state()[1], dt * (t + 1)))
+    rows = np.array(rows)
+    st = run.state()
+    assert np.array_equal(st[0], st[1])
+    ref = np.loadtxt(os.path.join(GOLDEN, "BKW8", "moments_BKW8.test.in"), comments="#")
+    m = np.abs(ref) > 1e-12
+    assert (np.abs(rows - ref)[m] / np.abs(ref)[m]).max() < 5e-7
+    orows = np.array(orc.run_0d(N, v, eta, W, f0, dt, 100, order=2))
+    m = np.abs(orows) > 1e-12
+    assert (np.abs(rows - orows)[m] / np.abs(orows)[m]).max() < 1e-10
+    run.close()
+    op.close()
+
+
+def test_run1d_heat_fixture(spb, orc):
+    """tests/heat_transport of the reference, whole steps on the GPU: upwindOne
+    with diffuse walls at T=1 and T=2, batched ComputeQ + projection, Euler."""
+    N, Lv, nX, dt, Kn = 8, 9.0, 250, 1e-4, 3.2
+    v, eta = orc.grid_1d(N, Lv)
+    W = orc.weights_load(os.path.join(GOLDEN, "heat_transport", "N8_isotropic_L_v9_lambda1.wts"), N)
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_weights(W)
+    x, dx = orc.uniform_mesh(nX, 1.0, 1)
+    f0 = np.tile(orc.maxwellian_1d(v, 1.0, 0.0, 1.5).ravel(), (nX, 1))
+    run = spb.Run.mesh(op, f0, x, dx, order=1, dt=dt, Kn=Kn, bc=spb.BC_DIFFUSE, T0=1.0, T1=2.0)
+    snaps = []
+    for t in range(10):
+        run.step(1)
+        m = run.moments()
+        snaps.append(np.stack([m[:, 0], m[:, 1], m[:, 4], m[:, 0] * m[:, 4]], axis=1))
+    snaps = np.array(snaps)
+    ref = np.loadtxt(os.path.join(GOLDEN, "heat_transport", "moments_heat_transport.test.in"),
+                     comments="#").reshape(11, nX, 6)[1:]
+    for c in (0, 2, 3):
+        assert (np.abs(ref[:, :, 2 + c] - snaps[:, :, c]) / np.abs(ref[:, :, 2 + c])).max() < 5e-7
+    osn = np.array(orc.run_1d(N, v, eta, W, f0, x, dx, dt, 10, 1, Kn, 1))[1:]
+    for c in (0, 2, 3):
+        assert (np.abs(osn[:, :, c] - snaps[:, :, c]) / np.abs(osn[:, :, c])).max() < 1e-10
+    run.close()
+    op.close()
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_run1d_order2_vs_oracle(spb, orc, bc):
+    """Order-2 1D steps (advectTwo, Heun collision step, advectTwo) against the
+    oracle's restatement of exec/boltz.c:285-354 on a shock-like profile."""
+    N, Lv, nX, dt, Kn, steps = 8, 5.0, 12, 0.002, 0.1, 4
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    W = op.get_weights()
+    x, dx = orc.uniform_mesh(nX, 1.0, 2)
+    f0 = _shock_cells(orc, v, nX)
+    run = spb.Run.mesh(op, f0, x, dx, order=2, dt=dt, Kn=Kn, bc=bc)
+    states = []
+    run_cb = lambda t, f: states.append(f[2:nX + 2].copy())  # noqa: E731
+    orc.run_1d(N, v, eta, W, f0, x, dx, dt, steps, 2, Kn, bc, callback=run_cb)
+    for t in range(steps):
+        run.step(1)
+        g = run.state()
+        assert np.abs(g - states[t]).max() < 1e-12 * np.abs(states[t]).max(), t
+    run.close()
+    op.close()
+
+
+@pytest.mark.parametrize("dim", [0, 1])
+def test_graph_replays_match_eager(spb, orc, dim):
+    """A captured step replayed several times, with eager work and synchronisations
+    between the replays, matches eager steps bit for bit after every replay."""
+    N, Lv = 8, 5.0
+    v, eta = (orc.grid_0d if dim == 0 else orc.grid_1d)(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    rng = np.random.default_rng(5)
+    if dim == 0:
+        f0 = np.stack([orc.maxwellian(v, 1.0, rng.uniform(-1, 1, 3), rng.uniform(0.6, 1.5)).ravel()
+                       + orc.maxwellian(v, 0.5, rng.uniform(-1, 1, 3), rng.uniform(0.6, 1.5)).ravel()
+                       for _ in range(6)])
+        mk = lambda: spb.Run.ensemble(op, f0, order=2, dt=0.01)  # noqa: E731
+    else:
+        nX = 10
+        f0 = _shock_cells(orc, v, nX)
+        x, dx = orc.uniform_mesh(nX, 1.0, 2)
+        mk = lambda: spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_DIFFUSE)  # noqa: E731
+    eager, graphed = mk(), mk()
+    graphed.use_graph(True)
+    other = op.compute_q(f0[:2])
+    for t in range(4):
+        eager.step(1)
+        graphed.step(1)
+        # eager work of the program between replays
+        assert np.array_equal(op.compute_q(f0[:2]), other)
+        assert np.array_equal(graphed.state(), eager.state()), t
+    graphed.step(3)
+    eager.step(3)
+    assert np.array_equal(graphed.state(), eager.state())
+    eager.close()
+    graphed.close()
+    op.close()
+
+
+def test_run0d_reduced_form_and_conservation(spb, orc):
+    """An ensemble advanced with FP32-streamed weights stays within 1e-6 of the
+    FP64 run, and both conserve mass, momentum and energy to 1e-12."""
+    N, Lv, dt, steps = 16, 6.0, 0.005, 5
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    from spectralbte_amd import workloads
+    f0 = workloads.two_maxwellian_ensemble(N, Lv, 8, seed=3)
+    r64 = spb.Run.ensemble(op, f0, order=2, dt=dt, form="fp64")
+    r32 = spb.Run.ensemble(op, f0, order=2, dt=dt, form="fp32")
+    r64.step(steps)
+    r32.step(steps)
+    a, b = r64.state(), r32.state()
+    for c in range(8):
+        d0 = np.linalg.norm(a[c] - f0[c])    # the change over the run
+        assert np.linalg.norm(b[c] - a[c]) < 1e-6 * d0
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    I, J, K = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    p = (w[I] * w[J] * w[K] * (v[1] - v[0]) ** 3).ravel()
+    C = np.stack([p, p * v[I].ravel(), p * v[J].ravel(), p * v[K].ravel(), p * (v[I] ** 2 + v[J] ** 2 + v[K] ** 2).ravel()])
+    for st in (a, b):
+        dm = np.abs((st - f0) @ C.T)
+        assert dm.max() < 1e-12 * (np.abs(f0) @ np.abs(C.T)).max()
+    r64.close(); r32.close(); op.close()
