@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-ref", action="store_true", help="skip the FP64 comparisons")
     p.add_argument("--no-extra", action="store_true", help="skip the secondary workload")
+    p.add_argument("--no-runs", action="store_true", help="skip the whole-step runs (shock, mixed ensemble)")
     p.add_argument("--cpu-cells", type=int, default=0, help="problems in the CPU sample (0: about 15 s of work)")
     return p.parse_args()
 
@@ -182,6 +183,10 @@ def main():
         extra["secondary"] = sec
         wl2.op.close()
 
+    if not a.no_extra and not a.no_runs:
+        extra["shock_run"] = shock_run(spb, a.form)
+        extra["ensemble_mixed_run"] = mixed_ensemble_run(spb, a.form)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(wl, a.cpu_cells)
@@ -197,6 +202,81 @@ def main():
            "roofline": roof, "cpu_baseline": cpu}
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def shock_run(spb, form, steps=3):
+    """Whole steps of the 1D-3V Mach-1.2 shock (256 cells, N=32, L_v=8, lambda=1,
+    Kn=0.1, order 2, dt=0.002): advectTwo, Heun collision step (2 Q per cell),
+    advectTwo, replayed as one captured HIP graph per step."""
+    from spectralbte_amd import workloads
+    N, Lv, nX = 32, 8.0, 256
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    x, dx = _mesh(nX, 1.0, 2)
+    f0 = workloads.shock_profile(N, Lv, nX)
+    run = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form=form)
+    op.autotune(nX, form, False)
+    run.use_graph(True)
+    run.step(1)
+    t0 = time.perf_counter()
+    run.step(steps)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    m = run.moments()
+    ok = bool(np.all(np.isfinite(m)))
+    run.close()
+    op.close()
+    log(f"shock run {form}: {ms:.2f} ms/step (graph), {2 * nX / ms * 1e3:.1f} Q/s")
+    return {"workload": "1D-3V Mach-1.2 shock, 256 cells, N=32, L_v=8, lambda=1, Kn=0.1, order 2 (advectTwo + "
+                        "Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
+            "form": form, "ms_per_step": round(ms, 3), "q_per_s": round(2 * nX / ms * 1e3, 1),
+            "steps_timed": steps, "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": ok}
+
+
+def _mesh(nX, length, order):
+    """Uniform cells on [0, length] with `order` ghost cells per side (src/mesh_setup.c:121-175)."""
+    d = length / nX
+    x = (np.arange(nX + 2 * order) - order + 0.5) * d
+    return x, np.full(nX + 2 * order, d)
+
+
+def mixed_ensemble_run(spb, form, ncell=512):
+    """The whole 0D ensemble: 512 problems at N=24 and 512 at N=32 (L_v=6,
+    lambda=1), Heun steps of dt=0.005 (2 Q per problem per step), the two grids'
+    runs queued on their own streams so they overlap."""
+    from spectralbte_amd import workloads
+    runs, ops = [], []
+    for N, seed in ((24, 0x5EED0024), (32, 0x5EED0032)):
+        v, eta = workloads.grid_0d(N, 6.0)
+        op = spb.Operator(N, 6.0, v, eta)
+        op.generate_weights_iso(1.0)
+        op.autotune(ncell, form, True)
+        f0 = workloads.two_maxwellian_ensemble(N, 6.0, ncell, seed=seed)
+        r = spb.Run.ensemble(op, f0, order=2, dt=0.005, Kn=1.0, form=form)
+        r.use_graph(True)
+        r.step(1)
+        runs.append(r)
+        ops.append(op)
+    steps = 2
+    t0 = time.perf_counter()
+    for r in runs:
+        r.step(steps, sync=False)
+    for op in ops:
+        op.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    per = {}
+    for N, r in zip((24, 32), runs):
+        t0 = time.perf_counter()
+        r.step(1)
+        per[f"N{N}_ms_per_step"] = round((time.perf_counter() - t0) * 1e3, 2)
+        r.close()
+    for op in ops:
+        op.close()
+    q = 2 * 2 * ncell / ms * 1e3
+    log(f"mixed ensemble {form}: {ms:.2f} ms/step for 2x{ncell} problems, {q:.1f} Q/s")
+    return {"workload": f"0D-3V ensemble, {ncell} problems at N=24 + {ncell} at N=32, L_v=6, lambda=1, Heun dt=0.005, "
+                        "ComputeQ_maxPreserve + projection, both grids' steps overlapped",
+            "form": form, "ms_per_step": round(ms, 2), "q_per_s": round(q, 1), **per}
 
 
 def cpu_baseline(wl, ncells):

@@ -32,14 +32,21 @@ struct SGeom {
   static constexpr int RW = 64 * TR;                 // rows per wave tile
   static constexpr int RWG = WAVES * RW;             // rows per workgroup
   static constexpr int N2 = N * N;
-  static constexpr int P = RWG >= N2 ? RWG / N2 : 1; // x-planes of output rows per workgroup
   static constexpr int NC4 = N * N * N / 4;          // 4-column groups per row
+  // x-planes of output rows a workgroup can touch: RWG / N2 when workgroups
+  // tile whole planes, 1 when they tile within a plane, else the most planes a
+  // run of RWG consecutive rows can straddle
+  static constexpr int P = (RWG % N2 == 0) ? RWG / N2 : ((N2 % RWG == 0) ? 1 : (RWG - 1) / N2 + 2);
   // cz steps per loop body: a multiple of TR (the circular window returns to its
   // start) and of 8 (an even number of 4-column W chunks)
   static constexpr int BS = (TR % 8 == 0) ? TR : ((TR % 4 == 0) ? 2 * TR : (TR % 2 == 0 ? 4 * TR : 8 * TR));
+  // a staged F line holds LW = N + BS values: entry m is F[(m - BS) mod N], so the
+  // window refill of every step of a body reads at (base - j + BS) with j the
+  // step's constant offset in the body, and never wraps
+  static constexpr int LW = N + BS;
   static_assert(N % BS == 0, "cz body must divide N");
   static_assert(N % TR == 0, "TR must divide N");
-  static_assert((RWG % N2 == 0) || (N2 % RWG == 0), "workgroup rows must tile x-planes");
+  static_assert((N * N * N) % RWG == 0, "workgroups must tile the output rows");
 };
 
 // Weight j (0..3) of a 16-byte unit pair of the permuted table, as a double.
@@ -71,22 +78,19 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   constexpr int UPW = (FORM == 0) ? 2 : 1;              // 16-byte units per 4 weights
   constexpr int BS = Gm::BS;                            // cz steps per loop body
 
-  __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N2];  // [p][b][plane][sy][sz]
-  __shared__ __attribute__((aligned(16))) double2 sG[NP * TB * N2];      // [p][b][cy][cz]
+  constexpr int LW = Gm::LW;
+  __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
+  __shared__ __attribute__((aligned(16))) double2 sG[NP * TB * N2];          // [p][b][cy][cz]
 
-  // ---- workgroup -> (r-tile, cell group), XCD-aware: consecutive workgroups
-  // on one XCD (bid mod 8) share the r-tile and so the W stream in its L2.
+  // ---- workgroup -> (r-tile, cell group), XCD-aware: workgroups are dispatched
+  // round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th contiguous
+  // slice of the r-tile-major list of (r-tile, cell group) pairs, and the
+  // workgroups it runs back to back share an r-tile and its W stream in its L2.
   const int ncg = (ncell + TB - 1) / TB;
-  const int bid = blockIdx.x;
-  int rt, cg;
-  if ((NRT & 7) == 0) {
-    const int xcd = bid & 7, q = bid >> 3;
-    cg = q % ncg;
-    rt = (q / ncg) * 8 + xcd;
-  } else {
-    cg = bid % ncg;
-    rt = bid / ncg;
-  }
+  const int total = NRT * ncg, per = (total + 7) >> 3;
+  const int vid = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if ((blockIdx.x >> 3) >= per || vid >= total) return;  // the few padding workgroups of the last slice
+  const int rt = vid / ncg, cg = vid - rt * ncg;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t wg_row0 = (size_t)rt * RWG;
   const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
@@ -126,14 +130,17 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 
   for (int cx = 0; cx < N; ++cx) {
     __syncthreads();
-    // ---- stage the shifted F x-planes of every cell, product and lane plane
-    for (int e = threadIdx.x; e < NP * TB * P * N2; e += 64 * WAVES) {
-      const int p = e / (TB * P * N2), r1 = e - p * (TB * P * N2);
-      const int b = r1 / (P * N2), r2 = r1 - b * (P * N2);
-      const int q = r2 / N2, yz = r2 - q * N2;
-      int sx = rx0 + q + N / 2 - cx;
+    // ---- stage the shifted F x-planes of every cell, product and lane plane,
+    // each z-line extended to LW values (entry m = F[(m - BS) mod N])
+    for (int e = threadIdx.x; e < NP * TB * P * N * LW; e += 64 * WAVES) {
+      const int ln = e / LW, m = e - ln * LW;                 // line (p, b, q, sy), entry
+      const int p = ln / (TB * P * N), r1 = ln - p * (TB * P * N);
+      const int b = r1 / (P * N), r2 = r1 - b * (P * N);
+      const int q = r2 / N, y = r2 - q * N;
+      int sx = (rx0 + q) % N + N / 2 - cx;
       sx = sx < 0 ? sx + N : (sx > N - 1 ? sx - N : sx);
-      sF[e] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + yz];
+      const int z = m < BS ? m - BS + N : m - BS;
+      sF[e] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
     }
     for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
       const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
@@ -148,28 +155,29 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * TB + b) * P + pl) * N + sy) * N;
+        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * TB + b) * P + pl) * N + sy) * LW;
       // circular window: at step cz, element t (row rz0 + t) is buf[(t - cz) mod TR]
-      // and holds F_line[(k0 + t - cz) mod N]
+      // and holds F_line[(k0 + t - cz) mod N] = line[BS + (k0 + t - cz) mod N]
       double2 buf[NP][TB][TR];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int b = 0; b < TB; ++b)
 #pragma unroll
-          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][((unsigned)(k0 + t)) % N];
+          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][BS + ((k0 + t) % N)];
       const int c4base = (cx * N + cy) * (N / 4);
       const double2 *gline = sG + (size_t)cy * N;  // G_b[cx][cy][:] of product p, cell b at + (p*TB+b)*N2
-      // One 4-step chunk: wc holds its weights, wn receives the next chunk's.
-      auto chunk = [&](uint4 (&wc)[TR][UPW], uint4 (&wn)[TR][UPW], int cz0, int c4n) __attribute__((always_inline)) {
+      // One 4-step chunk at body offset jb (a compile-time constant after
+      // unrolling): wc holds its weights, wn receives the next chunk's.  `fl`
+      // points at the refill entry of body step 0: line + BS + (k0 - 1 - cz_body) mod N.
+      auto chunk = [&](uint4 (&wc)[TR][UPW], uint4 (&wn)[TR][UPW], const double2 *const (&fl)[NP][TB],
+                       const double2 *gz, int jb, int c4n) __attribute__((always_inline)) {
         if (c4n < NC4) wload(c4n, wn);
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
-          const int cz = cz0 + jj;
-          const int j = cz % BS;  // position in the circular window period (cz0 % 4 == 0)
+          const int j = jb + jj;  // position in the body and in the window period
           // rows in descending order: row 0 reads the element loaded one step
           // earlier, last; the slot of row TR-1 is refilled right after its use
-          const unsigned idx = ((unsigned)(k0 + 2 * N - 1 - cz)) % N;
 #pragma unroll
           for (int t = TR - 1; t >= 0; --t) {
             const int sl = ((t - j) % TR + TR) % TR;
@@ -177,12 +185,12 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
               // complex products as explicit FMA chains: 4 ops per product, 2 for w
-              const double2 g0 = gline[(size_t)(0 * TB + b) * N2 + cz], f0 = buf[0][b][sl];
+              const double2 g0 = gz[(size_t)(0 * TB + b) * N2 + j], f0 = buf[0][b][sl];
               double pr = g0.x * f0.x, pi = g0.x * f0.y;
               pr = fma(-g0.y, f0.y, pr);
               pi = fma(g0.y, f0.x, pi);
               if constexpr (NP == 2) {
-                const double2 g1 = gline[(size_t)(1 * TB + b) * N2 + cz], f1 = buf[1][b][sl];
+                const double2 g1 = gz[(size_t)(1 * TB + b) * N2 + j], f1 = buf[1][b][sl];
                 pr = fma(g1.x, f1.x, pr);
                 pi = fma(g1.x, f1.y, pi);
                 pr = fma(-g1.y, f1.y, pr);
@@ -196,21 +204,30 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
               for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int b = 0; b < TB; ++b) buf[p][b][slnew] = line[p][b][idx];
+                for (int b = 0; b < TB; ++b) buf[p][b][slnew] = fl[p][b][-j];
             }
           }
         }
       };
       // The body is a whole number of window periods (TR) and of chunk pairs,
       // so window slots and the wA/wB roles are the same at every iteration.
+      int base = k0 - 1 + N;  // (k0 - 1 - cz_body) mod N, cz_body = it * BS
 #pragma unroll 1
       for (int it = 0; it < N / BS; ++it) {
+        base = base >= N ? base - N : base;
+        const double2 *fl[NP][TB];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+          for (int b = 0; b < TB; ++b) fl[p][b] = line[p][b] + BS + base;
+        const double2 *gz = gline + it * BS;
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
-          const int cz0 = it * BS + 4 * k, c4 = c4base + it * (BS / 4) + k;
-          chunk(wA, wB, cz0, c4 + 1);
-          chunk(wB, wA, cz0 + 4, c4 + 2);
+          const int c4 = c4base + it * (BS / 4) + k;
+          chunk(wA, wB, fl, gz, 4 * k, c4 + 1);
+          chunk(wB, wA, fl, gz, 4 * k + 4, c4 + 2);
         }
+        base = base - BS + N;
       }
     }
   }
@@ -237,7 +254,8 @@ int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, hi
   using Gm = SGeom<N, TR, WAVES>;
   const int ncg = (a->ncell + TB - 1) / TB;
   const int nrt = (int)((size_t)N * N * N / Gm::RWG);
-  hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM>), dim3(nrt * ncg), dim3(64 * WAVES), 0, s,
+  const int per = (nrt * ncg + 7) / 8;  // workgroups per XCD slice
+  hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM>), dim3(8 * per), dim3(64 * WAVES), 0, s,
                      (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
                      (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell);
   return (int)hipGetLastError();
@@ -256,8 +274,10 @@ const SEntry kSTable[] = {
     // N = 32: TR=8 (4 lanes per z-line, 2 waves per x-plane) and TR=4 (8 lanes per z-line)
     S3(32, 8, 2, 1, 1), S3(32, 8, 2, 2, 1), S3(32, 4, 4, 2, 1), S3(32, 4, 2, 2, 1), S3(32, 4, 4, 4, 1),
     S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S3(32, 4, 4, 2, 2), S3(32, 4, 4, 1, 2),
-    // N = 24: TR=6 (3 waves = 2 x-planes), TR=4 (9 waves = 4 x-planes)
-    S3(24, 6, 3, 1, 1), S3(24, 6, 3, 2, 1), S3(24, 6, 3, 1, 2), S3(24, 4, 9, 1, 1), S3(24, 4, 9, 2, 1),
+    // N = 24 (rows per workgroup must divide 24^3): TR=3 x 3 waves = one x-plane,
+    // TR=6 x 3 waves = 2 planes, TR=4 x 2 waves and TR=8 x 1 wave straddle planes
+    S3(24, 3, 3, 1, 1), S3(24, 3, 3, 2, 1), S3(24, 3, 3, 1, 2), S3(24, 6, 3, 1, 1), S3(24, 6, 3, 1, 2),
+    S3(24, 4, 2, 1, 1), S3(24, 4, 2, 2, 1), S3(24, 4, 2, 1, 2), S3(24, 8, 1, 1, 1), S3(24, 8, 1, 1, 2),
     S3(24, 4, 9, 1, 2),
     // N = 16: TR=4 (1 wave = 1 x-plane)
     S3(16, 4, 1, 1, 1), S3(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S3(16, 4, 1, 2, 2),

@@ -542,12 +542,18 @@ static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
  * workgroups (one r-tile is a whole x-plane of output modes). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   if (form == SPBTE_FORM_FP32A) return 0;
-  static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 2, 1}, {4, 4, 1, 1}};
+  /* measured fastest on 256-512 cells (MI355X): FP64 weights favour 8 rows per
+   * lane for the two-product sum, FP32 weights 4 rows and 4 waves */
+  static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}}, s32m64[] = {{8, 2, 1, 1}};
+  static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
   const spbte_tile *c = NULL;
   int nc = 0;
-  if (N == 32) { c = nprod == 2 ? s32m : s32; nc = 2; }
-  else if (N == 24) return 0; /* the 9-wave N=24 tiles cap VGPRs at 168 and spill: tiled kernel */
+  if (N == 32) {
+    if (nprod == 1) { c = s32; nc = 2; }
+    else if (form == SPBTE_FORM_FP64) { c = s32m64; nc = 1; }
+    else { c = s32m; nc = 1; }
+  } else if (N == 24) { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
   else if (N == 16) { c = nprod == 2 ? s16m : s16; nc = 2; }
   else return 0;
   for (int i = 0; i < nc; i++) {
@@ -783,8 +789,8 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
 static const spbte_tile k_cand[] = {
     {4, 1, 1, 0}, {4, 2, 1, 0}, {4, 2, 2, 0}, {4, 4, 2, 0}, {4, 2, 4, 0}, {4, 4, 4, 0}, {8, 4, 2, 0}, {8, 8, 2, 0},
     {8, 4, 4, 0}, {6, 8, 1, 0}, {6, 8, 2, 0}, {6, 8, 4, 0}, {4, 8, 2, 0}, {4, 8, 4, 0}, {4, 4, 1, 0}, {2, 8, 2, 0},
-    {8, 2, 1, 1}, {8, 2, 2, 1}, {4, 4, 2, 1}, {4, 2, 2, 1}, {4, 4, 4, 1}, {4, 4, 1, 1}, {6, 3, 1, 1}, {6, 3, 2, 1},
-    {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1}, {4, 9, 2, 1}};
+    {8, 2, 1, 1}, {8, 2, 2, 1}, {4, 4, 2, 1}, {4, 2, 2, 1}, {4, 4, 4, 1}, {4, 4, 1, 1}, {6, 3, 1, 1}, {3, 3, 1, 1},
+    {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1}};
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");

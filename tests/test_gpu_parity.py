@@ -90,10 +90,34 @@ def test_qhat_tiles_agree(spb, orc, N):
     op.set_weights(W)
     ref = np.stack([orc.qhat(N, W, F[b], G[b]).ravel() for b in range(ncell)])
     tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 8, 2, 0), (4, 8, 4, 0), (4, 1, 1, 1), (4, 1, 2, 1)],
-             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 8, 4, 0), (6, 3, 1, 1), (6, 3, 2, 1)]}[N]
+             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 8, 4, 0), (6, 3, 1, 1), (3, 3, 1, 1), (3, 3, 2, 1), (4, 2, 1, 1),
+                  (4, 2, 2, 1), (8, 1, 1, 1)]}[N]
     for t in tiles:
         op.set_tile("fp64", False, *t)
         assert rel_l2(op.qhat(F, G, "fp64"), ref) < 1e-12, t
+
+
+@pytest.mark.parametrize("N", [16, 24])
+def test_maxpreserve_tiles_agree(spb, orc, N):
+    """Every instantiated two-product (fused maxPreserve) tile against the oracle's
+    three-pass ComputeQ_maxPreserve, in FP64 and with FP32-streamed weights."""
+    Lv = 6.0
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    W = op.get_weights()
+    rng = np.random.default_rng(N + 1)
+    ncell = 3
+    f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(ncell)])
+    ref = np.stack([orc.compute_q_maxpreserve(N, v, eta, W, f[b], f[b]).ravel() for b in range(ncell)])
+    tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 1, 1, 1), (4, 1, 2, 1)],
+             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 3, 1, 1), (3, 3, 1, 1), (4, 2, 1, 1), (8, 1, 1, 1), (4, 9, 1, 1)]}[N]
+    for t in tiles:
+        for form, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
+            op.set_tile(form, True, *t)
+            Q = op.compute_q(f, form=form, maxpreserve=True)
+            for b in range(ncell):
+                assert rel_l2(Q[b], ref[b]) < tol, (t, form, b)
 
 
 # --------------------------------------------------- reference interface
