@@ -169,6 +169,50 @@ int spbte_run_moments(spbte_run *run, double *out);
 int spbte_advect(spbte_op *op, double *d_f, double *d_fc, int nX, int order, const double *x, const double *dx,
                  double dt, int bc, double T0, double T1);
 
+/* ---------------------------------------------------------------------------
+ * Learned collision surrogate (no counterpart in the reference).  A spectral
+ * mixing network maps the block of Kb^3 lowest transform modes of a cell to
+ * the same block of Qhat (features (T(f - M_f), T(M_f)) / rho; output
+ * Qhat / rho^2); its prediction is zero-padded, inverse-transformed and passed
+ * through the FP64 projection.  Layers: low-rank mode mixing H += U (V^T H)
+ * (rank `rank`) and a GELU channel MLP of `width` -> 2 width -> `width`
+ * channels.  GEMMs run on BF16 MFMA with FP32 accumulation; parameters, Adam
+ * moments and activations are FP32.  Targets are the operator's exact Qhat
+ * (ComputeQ, or the fused maxPreserve sum when `maxpreserve`), computed with
+ * the FP64 table the operator holds.  Loss: mean over cells of
+ * |y - t|^2 / |t|^2 on the block.  The network shares the operator's stream
+ * and must not outlive it. */
+typedef struct spbte_net spbte_net;
+
+int spbte_net_create(spbte_net **net, spbte_op *op, int Kb, int width, int rank, int layers, int maxpreserve,
+                     unsigned long long seed);
+void spbte_net_destroy(spbte_net *net);
+long spbte_net_num_params(spbte_net *net);
+/* offsets of P, pb, E, {U, V, W1, b1, W2, b2} per layer, O, ob in the parameter vector; returns their count */
+int spbte_net_layout(spbte_net *net, long *offsets);
+int spbte_net_get_params(spbte_net *net, float *params);
+int spbte_net_set_params(spbte_net *net, const float *params);
+int spbte_net_set_scales(spbte_net *net, double fscale, double tscale);
+int spbte_net_get_scales(spbte_net *net, double *fscale, double *tscale);
+/* Feature and target scales from ncell cells (d_Q: ncell * N^3 device scratch). */
+int spbte_net_calibrate(spbte_net *net, const double *d_f, double *d_Q, int ncell);
+/* Training set in HBM: reserve, then append (features, exact targets) of cells. */
+int spbte_net_dataset_reserve(spbte_net *net, long nsamples);
+int spbte_net_dataset_add(spbte_net *net, const double *d_f, double *d_Q, int ncell);
+long spbte_net_dataset_size(spbte_net *net);
+/* nsteps Adam steps on shuffled batches of the dataset; losses[t] if non-null. */
+int spbte_net_train(spbte_net *net, int nsteps, int batch, float lr, unsigned long long seed, float *losses);
+/* Loss and gradient (host copy) of dataset samples idx[0..B), no update. */
+int spbte_net_grad(spbte_net *net, const int *idx, int B, float *grad, double *loss);
+/* Relative L2 error of the block prediction for dataset samples. */
+int spbte_net_eval(spbte_net *net, const int *idx, int B, float *err);
+/* Network alone on host features [m][b][4] -> host outputs [m][b][2]; and the
+ * loss gradient for host features and targets [m][b][2]. */
+int spbte_net_forward_features(spbte_net *net, const float *X, int B, float *Y);
+int spbte_net_grad_features(spbte_net *net, const float *X, const float *T, int B, float *grad, double *loss);
+/* Surrogate Q of ncell cells (device arrays), optionally projected. */
+int spbte_net_predict(spbte_net *net, const double *d_f, double *d_Q, int ncell, int conserve);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@ MAXPRESERVE, CONSERVE, SELF = 1, 2, 4
 
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -110,6 +111,24 @@ def lib():
             "spbte_run_step": (I, [_vp, I]),
             "spbte_run_moments": (I, [_vp, _dp]),
             "spbte_advect": (I, [_vp, _vp, _vp, I, I, _dp, _dp, D, I, D, D]),
+            "spbte_net_create": (I, [ctypes.POINTER(_vp), _vp, I, I, I, I, I, ctypes.c_ulonglong]),
+            "spbte_net_destroy": (None, [_vp]),
+            "spbte_net_num_params": (ctypes.c_long, [_vp]),
+            "spbte_net_layout": (I, [_vp, ctypes.POINTER(ctypes.c_long)]),
+            "spbte_net_get_params": (I, [_vp, _fp]),
+            "spbte_net_set_params": (I, [_vp, _fp]),
+            "spbte_net_set_scales": (I, [_vp, D, D]),
+            "spbte_net_get_scales": (I, [_vp, _dp, _dp]),
+            "spbte_net_calibrate": (I, [_vp, _vp, _vp, I]),
+            "spbte_net_dataset_reserve": (I, [_vp, ctypes.c_long]),
+            "spbte_net_dataset_add": (I, [_vp, _vp, _vp, I]),
+            "spbte_net_dataset_size": (ctypes.c_long, [_vp]),
+            "spbte_net_train": (I, [_vp, I, I, ctypes.c_float, ctypes.c_ulonglong, _fp]),
+            "spbte_net_grad": (I, [_vp, ip, I, _fp, _dp]),
+            "spbte_net_eval": (I, [_vp, ip, I, _fp]),
+            "spbte_net_forward_features": (I, [_vp, _fp, I, _fp]),
+            "spbte_net_grad_features": (I, [_vp, _fp, _fp, I, _fp, _dp]),
+            "spbte_net_predict": (I, [_vp, _vp, _vp, I, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -391,6 +410,118 @@ def advect(op, f, x, dx, order, dt, bc, T0=1.0, T1=2.0):
     d_fc = DeviceArray(f.shape, np.zeros_like(f))
     _check(lib().spbte_advect(op.h, d_f.ptr, d_fc.ptr, nX, order, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1))
     return d_f.download(), d_fc.download()
+
+
+# ------------------------------------------------------------ learned surrogate
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+class Net:
+    """spbte_net: the spectral mixing surrogate of the collision operator of
+    ``op`` on the kept block of Kb^3 modes (see include/spbte.h)."""
+
+    def __init__(self, op, Kb, width=64, rank=128, layers=4, maxpreserve=False, seed=1):
+        self.op = op
+        self.Kb, self.C, self.r, self.L = Kb, width, rank, layers
+        self.M = Kb ** 3
+        self.h = _vp()
+        _check(lib().spbte_net_create(ctypes.byref(self.h), op.h, Kb, width, rank, layers, int(maxpreserve), seed))
+        self.npar = lib().spbte_net_num_params(self.h)
+
+    def params(self):
+        out = np.empty(self.npar, dtype=np.float32)
+        _check(lib().spbte_net_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.npar
+        _check(lib().spbte_net_set_params(self.h, _fptr(p)))
+
+    def offsets(self):
+        o = (ctypes.c_long * (3 + 6 * self.L + 2))()
+        k = lib().spbte_net_layout(self.h, o)
+        return list(o)[:k]
+
+    def scales(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().spbte_net_get_scales(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_scales(self, fscale, tscale):
+        _check(lib().spbte_net_set_scales(self.h, fscale, tscale))
+
+    def calibrate(self, d_f, d_Q, ncell):
+        _check(lib().spbte_net_calibrate(self.h, d_f, d_Q, int(ncell)))
+
+    def reserve(self, nsamples):
+        _check(lib().spbte_net_dataset_reserve(self.h, int(nsamples)))
+
+    def add(self, d_f, d_Q, ncell):
+        _check(lib().spbte_net_dataset_add(self.h, d_f, d_Q, int(ncell)))
+
+    def dataset_size(self):
+        return lib().spbte_net_dataset_size(self.h)
+
+    def train(self, nsteps, batch, lr, seed=7):
+        losses = np.zeros(nsteps, dtype=np.float32)
+        _check(lib().spbte_net_train(self.h, int(nsteps), int(batch), float(lr), seed, _fptr(losses)))
+        return losses
+
+    def grad(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.empty(self.npar, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().spbte_net_grad(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), idx.size, _fptr(g),
+                                    ctypes.byref(loss)))
+        return loss.value, g
+
+    def eval(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        e = np.empty(idx.size, dtype=np.float32)
+        _check(lib().spbte_net_eval(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), idx.size, _fptr(e)))
+        return e
+
+    def forward_features(self, X):
+        X = _f32(X)
+        B = X.shape[1]
+        Y = np.empty((self.M, B, 2), dtype=np.float32)
+        _check(lib().spbte_net_forward_features(self.h, _fptr(X), B, _fptr(Y)))
+        return Y
+
+    def grad_features(self, X, T):
+        X, T = _f32(X), _f32(T)
+        g = np.empty(self.npar, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().spbte_net_grad_features(self.h, _fptr(X), _fptr(T), X.shape[1], _fptr(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def predict_dev(self, d_f, d_Q, ncell, conserve=True):
+        _check(lib().spbte_net_predict(self.h, d_f, d_Q, int(ncell), int(conserve)))
+
+    def predict(self, f, conserve=True):
+        f = _c(f).reshape(-1, self.op.N ** 3)
+        d_f = DeviceArray.from_host(f)
+        d_Q = DeviceArray(f.shape)
+        self.predict_dev(d_f.ptr, d_Q.ptr, f.shape[0], conserve)
+        self.op.sync()
+        return d_Q.download()
+
+    def close(self):
+        if self.h:
+            lib().spbte_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------ reference interface (numpy)

@@ -1,0 +1,520 @@
+// k_net.hip -- kernels of the learned collision surrogate: a BF16 MFMA GEMM with
+// FP32 accumulation (forward and backward products of the network), and the
+// elementwise kernels around it (spectral features, lift, GELU, head, loss,
+// column sums, Adam).  No counterpart in the reference.
+//
+// GEMM: C[i][j] (+)= sum_k A(i,k) B(k,j), A(i,k) = A[i*sa0 + k*sa1],
+// B(k,j) = B[k*sb0 + j*sb1], FP32 in global memory.  Operands are rounded to
+// BF16 (round to nearest even) on their way into LDS and multiplied with
+// v_mfma_f32_16x16x32_bf16; sums are FP32.  A workgroup computes a 64x64 tile
+// with 4 waves (32x32 each, 2x2 MFMA tiles); K advances 32 per LDS stage.
+// Split-K: grid.z slices of K write FP32 partial tiles that k_splitk_reduce
+// sums with the epilogue.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "spbte_internal.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 32;
+constexpr int KP = TK + 8;  // padded k extent of an LDS row (bf16): 80-byte rows
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+struct GemmArgs {
+  const float *A, *B;
+  float *C;           // output (or the split-K partials)
+  const float *bias;  // per column j, or null
+  float *G;           // GELU(C) output (epilogue 1), or null
+  long sa0, sa1, sb0, sb1, sc0, sc1;
+  int M, N, K;
+  int accumulate;     // C += result instead of C = result
+  int kslice;         // K per split (multiple of TK)
+};
+
+// Stage a 64x32 tile of A (rows i0.., cols k0..) as sA[i][k] and a 32x64 tile
+// of B as sB[j][k], both k-contiguous, rounding to BF16.  Out-of-range
+// elements are zero.  Each of the 256 threads moves 8 elements of each.
+__device__ __forceinline__ void stage(const GemmArgs &g, __bf16 (*sA)[KP], __bf16 (*sB)[KP], int i0, int j0, int k0,
+                                      int kend) {
+  const int t = threadIdx.x;
+  if (g.sa1 == 1) {  // A row-major: thread reads 8 consecutive k of one row
+    const int i = t >> 2, kk = (t & 3) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int gi = i0 + i, gk = k0 + kk + e;
+      sA[i][kk + e] = (__bf16)((gi < g.M && gk < kend) ? g.A[gi * g.sa0 + gk] : 0.0f);
+    }
+  } else {           // A column-major: thread reads 8 consecutive i of one k
+    const int kk = t >> 3, i = (t & 7) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int gi = i0 + i + e, gk = k0 + kk;
+      sA[i + e][kk] = (__bf16)((gi < g.M && gk < kend) ? g.A[gi * g.sa0 + gk * g.sa1] : 0.0f);
+    }
+  }
+  if (g.sb0 == 1) {  // B stored with k contiguous (column-major): 8 consecutive k of one j
+    const int j = t >> 2, kk = (t & 3) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int gj = j0 + j, gk = k0 + kk + e;
+      sB[j][kk + e] = (__bf16)((gj < g.N && gk < kend) ? g.B[gk + gj * g.sb1] : 0.0f);
+    }
+  } else {           // B row-major: 8 consecutive j of one k
+    const int kk = t >> 3, j = (t & 7) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int gj = j0 + j + e, gk = k0 + kk;
+      sB[j + e][kk] = (__bf16)((gj < g.N && gk < kend) ? g.B[gk * g.sb0 + gj * g.sb1] : 0.0f);
+    }
+  }
+}
+
+// epi: 0 plain (C = / += acc), 1 bias + GELU copy.  split: write partials.
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g, int split) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[TM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[TN][KP];
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  const int kbeg = blockIdx.z * g.kslice;
+  const int kend = min(g.K, kbeg + g.kslice);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
+    __syncthreads();
+    stage(g, sA, sB, i0, j0, k0, kend);
+    __syncthreads();
+    // lane l: row/col (l % 16), k group 8 (l / 16) .. + 8
+    const int r = lane & 15, kg = (lane >> 4) * 8;
+    bf16x8 av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) av[a] = *(const bf16x8 *)&sA[wi + 16 * a + r][kg];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x8 *)&sB[wj + 16 * b + r][kg];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+  // D layout: lane l holds rows 4 (l / 16) + q, column l % 16 of each 16x16 tile
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + wi + 16 * a + 4 * (lane >> 4) + q, j = j0 + wj + 16 * b + (lane & 15);
+        if (i >= g.M || j >= g.N) continue;
+        const float v = acc[a][b][q];
+        if (split) {
+          g.C[((long)blockIdx.z * g.M + i) * g.N + j] = v;  // partials: [z][i][j]
+          continue;
+        }
+        float *c = g.C + i * g.sc0 + j * g.sc1;
+        float o = g.accumulate ? *c + v : v;
+        if (g.bias) o += g.bias[j];
+        *c = o;
+        if (g.G) g.G[i * g.sc0 + j * g.sc1] = gelu(o);
+      }
+}
+
+__global__ void __launch_bounds__(256) k_splitk_reduce(GemmArgs g, const float *__restrict__ part, int nsplit) {
+  const long n = (long)g.M * g.N;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const int i = (int)(e / g.N), j = (int)(e - (long)i * g.N);
+    float v = 0.f;
+    for (int z = 0; z < nsplit; ++z) v += part[(long)z * n + e];
+    float *c = g.C + i * g.sc0 + j * g.sc1;
+    float o = g.accumulate ? *c + v : v;
+    if (g.bias) o += g.bias[j];
+    *c = o;
+    if (g.G) g.G[i * g.sc0 + j * g.sc1] = gelu(o);
+  }
+}
+
+// ---- elementwise kernels -------------------------------------------------
+// dZ = dG * GELU'(Z)
+__global__ void __launch_bounds__(256) k_gelu_bwd(long n, const float *__restrict__ Z, const float *__restrict__ dG,
+                                                  float *__restrict__ dZ) {
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) dZ[e] = dG[e] * gelu_grad(Z[e]);
+}
+
+// Column sums of X (rows x cols, row-major) into partial[blk][col]; second stage sums blocks.
+__global__ void __launch_bounds__(256) k_colsum_part(long rows, int cols, const float *__restrict__ X, float *__restrict__ part,
+                                                     long rows_per_blk) {
+  const long r0 = (long)blockIdx.x * rows_per_blk, r1 = min(rows, r0 + rows_per_blk);
+  for (int c = threadIdx.x; c < cols; c += 256) {
+    float s = 0.f;
+    for (long r = r0; r < r1; ++r) s += X[r * cols + c];
+    part[(long)blockIdx.x * cols + c] = s;
+  }
+}
+__global__ void __launch_bounds__(256) k_colsum_final(int nblk, int cols, const float *__restrict__ part, float *out,
+                                                      int accumulate) {
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) {
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += part[(long)b * cols + c];
+    out[c] = accumulate ? out[c] + s : s;
+  }
+}
+
+// Features of the kept mode block (K^3 modes about eta = 0) of each cell, mode-major:
+// X[m][b][0..3] = (T(f - M)/rho, T(M)/rho)[mode m] * fscale, from the complex
+// transforms Fd (f - M) and Fm (M) of the cells (N^3 double2 each).
+__global__ void __launch_bounds__(256) k_net_features(int N, int Kb, int ncell, const double2 *__restrict__ Fd,
+                                                      const double2 *__restrict__ Fm, const double *__restrict__ mom,
+                                                      double fscale, float *__restrict__ X) {
+  const long n3 = (long)N * N * N, M = (long)Kb * Kb * Kb;
+  const long total = M * ncell;
+  const int lo = N / 2 - Kb / 2;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long m = e / ncell;
+    const int b = (int)(e - m * ncell);
+    const int mz = (int)(m % Kb), my = (int)((m / Kb) % Kb), mx = (int)(m / ((long)Kb * Kb));
+    const long idx = ((long)(lo + mx) * N + (lo + my)) * N + (lo + mz);
+    const double s = fscale / mom[(long)b * SPBTE_MOM_STRIDE];
+    const double2 d = Fd[(long)b * n3 + idx], q = Fm[(long)b * n3 + idx];
+    float *x = X + e * 4;
+    x[0] = (float)(d.x * s);
+    x[1] = (float)(d.y * s);
+    x[2] = (float)(q.x * s);
+    x[3] = (float)(q.y * s);
+  }
+}
+
+// Targets: T[m][b][0..1] = Qhat[mode m] / rho^2 * tscale.
+__global__ void __launch_bounds__(256) k_net_targets(int N, int Kb, int ncell, const double2 *__restrict__ Qh,
+                                                     const double *__restrict__ mom, double tscale, float *__restrict__ T) {
+  const long n3 = (long)N * N * N, M = (long)Kb * Kb * Kb;
+  const int lo = N / 2 - Kb / 2;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < M * ncell; e += (long)gridDim.x * 256) {
+    const long m = e / ncell;
+    const int b = (int)(e - m * ncell);
+    const int mz = (int)(m % Kb), my = (int)((m / Kb) % Kb), mx = (int)(m / ((long)Kb * Kb));
+    const long idx = ((long)(lo + mx) * N + (lo + my)) * N + (lo + mz);
+    const double rho = mom[(long)b * SPBTE_MOM_STRIDE];
+    const double2 q = Qh[(long)b * n3 + idx];
+    T[e * 2] = (float)(q.x * tscale / (rho * rho));
+    T[e * 2 + 1] = (float)(q.y * tscale / (rho * rho));
+  }
+}
+
+// Network output -> zero-padded N^3 spectrum of each cell: Qh = Y * rho^2 / tscale on the block.
+__global__ void __launch_bounds__(256) k_net_scatter(int N, int Kb, int ncell, const float *__restrict__ Y,
+                                                     const double *__restrict__ mom, double tscale, double2 *__restrict__ Qh) {
+  const long n3 = (long)N * N * N, M = (long)Kb * Kb * Kb;
+  const int lo = N / 2 - Kb / 2;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n3 * ncell; e += (long)gridDim.x * 256) {
+    const int b = (int)(e / n3);
+    const long idx = e - (long)b * n3;
+    const int iz = (int)(idx % N) - lo, iy = (int)((idx / N) % N) - lo, ix = (int)(idx / ((long)N * N)) - lo;
+    double2 out = make_double2(0.0, 0.0);
+    if (ix >= 0 && ix < Kb && iy >= 0 && iy < Kb && iz >= 0 && iz < Kb) {
+      const long m = ((long)ix * Kb + iy) * Kb + iz;
+      const double rho = mom[(long)b * SPBTE_MOM_STRIDE];
+      const double s = rho * rho / tscale;
+      out = make_double2((double)Y[(m * ncell + b) * 2] * s, (double)Y[(m * ncell + b) * 2 + 1] * s);
+    }
+    Qh[e] = out;
+  }
+}
+
+// Lift: H[m][b][c] = sum_q X[m][b][q] P[q][c] + E[m][c] + pb[c]   (C channels, 4 features)
+__global__ void __launch_bounds__(256) k_net_lift(long M, int ncell, int C, const float *__restrict__ X,
+                                                  const float *__restrict__ P, const float *__restrict__ E,
+                                                  const float *__restrict__ pb, float *__restrict__ H) {
+  const long total = M * ncell * C;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int c = (int)(e % C);
+    const long mb = e / C, m = mb / ncell;
+    const float *x = X + mb * 4;
+    H[e] = x[0] * P[c] + x[1] * P[C + c] + x[2] * P[2 * C + c] + x[3] * P[3 * C + c] + E[m * C + c] + pb[c];
+  }
+}
+
+// Lift backward: dP[q][c] and dpb[c] need sums over (m, b); dE[m][c] = sum_b dH[m][b][c].
+// One workgroup per mode m: dE directly, and per-mode partials of dP, dpb.
+__global__ void __launch_bounds__(256) k_net_lift_bwd(long M, int ncell, int C, const float *__restrict__ X,
+                                                      const float *__restrict__ dH, float *__restrict__ dE,
+                                                      float *__restrict__ part /* [M][5*C] */) {
+  const long m = blockIdx.x;
+  for (int c = threadIdx.x; c < C; c += 256) {
+    float se = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    for (int b = 0; b < ncell; ++b) {
+      const float d = dH[(m * ncell + b) * C + c];
+      const float *x = X + (m * ncell + b) * 4;
+      se += d;
+      s0 += d * x[0];
+      s1 += d * x[1];
+      s2 += d * x[2];
+      s3 += d * x[3];
+    }
+    dE[m * C + c] = se;
+    float *p = part + m * 5 * C;
+    p[c] = s0;
+    p[C + c] = s1;
+    p[2 * C + c] = s2;
+    p[3 * C + c] = s3;
+    p[4 * C + c] = se;
+  }
+}
+
+// Head: Y[r][0..1] = sum_c H[r][c] O[c][0..1] + ob   (r = (m, b))
+__global__ void __launch_bounds__(256) k_net_head(long R, int C, const float *__restrict__ H, const float *__restrict__ O,
+                                                  const float *__restrict__ ob, float *__restrict__ Y) {
+  for (long r = (long)blockIdx.x * 256 + threadIdx.x; r < R; r += (long)gridDim.x * 256) {
+    float y0 = ob[0], y1 = ob[1];
+    const float *h = H + r * C;
+    for (int c = 0; c < C; ++c) {
+      y0 = fmaf(h[c], O[2 * c], y0);
+      y1 = fmaf(h[c], O[2 * c + 1], y1);
+    }
+    Y[2 * r] = y0;
+    Y[2 * r + 1] = y1;
+  }
+}
+
+// Head backward: dH[r][c] = dY[r] . O[c]; per-block partials of dO[c][0..1] and dob.
+__global__ void __launch_bounds__(256) k_net_head_bwd(long R, int C, const float *__restrict__ H,
+                                                      const float *__restrict__ O, const float *__restrict__ dY,
+                                                      float *__restrict__ dH, float *__restrict__ part /* [blk][2C+2] */,
+                                                      long rows_per_blk) {
+  const long r0 = (long)blockIdx.x * rows_per_blk, r1 = min(R, r0 + rows_per_blk);
+  for (long r = r0 + threadIdx.x / C; r < r1; r += 256 / C) {
+    const int c = threadIdx.x % C;
+    dH[r * C + c] = dY[2 * r] * O[2 * c] + dY[2 * r + 1] * O[2 * c + 1];
+  }
+  for (int c = threadIdx.x; c < C + 1; c += 256) {
+    float s0 = 0.f, s1 = 0.f;
+    for (long r = r0; r < r1; ++r) {
+      const float h = c < C ? H[r * C + c] : 1.0f;
+      s0 += h * dY[2 * r];
+      s1 += h * dY[2 * r + 1];
+    }
+    part[(long)blockIdx.x * (2 * C + 2) + 2 * c] = s0;
+    part[(long)blockIdx.x * (2 * C + 2) + 2 * c + 1] = s1;
+  }
+}
+
+// Per-cell relative squared error and the gradient of the mean over cells:
+// L_b = sum_m |Y - T|^2 / (sum_m |T|^2 + eps),  dY = 2 (Y - T) / (den_b * ncell).
+__global__ void __launch_bounds__(256) k_net_loss(long M, int ncell, const float *__restrict__ Y, const float *__restrict__ T,
+                                                  float *__restrict__ dY, float *__restrict__ loss_b, float eps) {
+  __shared__ float red[2][4];
+  const int b = blockIdx.x;
+  float num = 0.f, den = 0.f;
+  for (long m = threadIdx.x; m < M; m += 256) {
+    const long r = m * ncell + b;
+    const float d0 = Y[2 * r] - T[2 * r], d1 = Y[2 * r + 1] - T[2 * r + 1];
+    num += d0 * d0 + d1 * d1;
+    den += T[2 * r] * T[2 * r] + T[2 * r + 1] * T[2 * r + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    num += __shfl_down(num, off, 64);
+    den += __shfl_down(den, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = num; red[1][threadIdx.x >> 6] = den; }
+  __syncthreads();
+  num = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+  den = red[1][0] + red[1][1] + red[1][2] + red[1][3] + eps;
+  if (threadIdx.x == 0) loss_b[b] = num / den;
+  if (dY) {
+    const float s = 2.0f / (den * (float)ncell);
+    for (long m = threadIdx.x; m < M; m += 256) {
+      const long r = m * ncell + b;
+      dY[2 * r] = s * (Y[2 * r] - T[2 * r]);
+      dY[2 * r + 1] = s * (Y[2 * r + 1] - T[2 * r + 1]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adam(long n, float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m1, float *__restrict__ m2, float lr, float b1,
+                                              float b2, float eps, float c1, float c2, float wd) {
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const float gr = g[e];
+    const float a = b1 * m1[e] + (1.f - b1) * gr, v = b2 * m2[e] + (1.f - b2) * gr * gr;
+    m1[e] = a;
+    m2[e] = v;
+    p[e] -= lr * ((a / c1) / (sqrtf(v / c2) + eps) + wd * p[e]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_axpy_f(long n, float a, const float *__restrict__ x, float *__restrict__ y) {
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) y[e] += a * x[e];
+}
+
+inline unsigned grid_for(long n) {
+  long b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------
+extern "C" int spbte_net_gemm(const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0,
+                              long sc1, int M, int N, int K, int accumulate, const float *bias, float *G, float *work,
+                              size_t work_floats, hipStream_t s) {
+  if (M <= 0 || N <= 0) return 0;
+  if (!((sa1 == 1) || (sa0 == 1)) || !((sb0 == 1) || (sb1 == 1))) return (int)hipErrorInvalidValue;
+  GemmArgs g = {A, B, C, bias, G, sa0, sa1, sb0, sb1, sc0, sc1, M, N, K, accumulate, K};
+  const unsigned gx = (N + TN - 1) / TN, gy = (M + TM - 1) / TM;
+  // split K when the output tile grid alone cannot fill the chip
+  int nsplit = 1;
+  const long tiles = (long)gx * gy;
+  if (tiles < 512 && K > 4 * TK && work) {
+    nsplit = (int)((1024 + tiles - 1) / tiles);
+    const int maxs = K / (2 * TK);
+    if (nsplit > maxs) nsplit = maxs;
+    while (nsplit > 1 && (size_t)nsplit * M * N > work_floats) nsplit--;
+  }
+  if (nsplit <= 1) {
+    g.kslice = ((K + TK - 1) / TK) * TK;
+    hipLaunchKernelGGL(k_gemm, dim3(gx, gy, 1), dim3(256), 0, s, g, 0);
+    return (int)hipGetLastError();
+  }
+  const int kt = (K + TK - 1) / TK;
+  g.kslice = ((kt + nsplit - 1) / nsplit) * TK;
+  nsplit = (K + g.kslice - 1) / g.kslice;
+  GemmArgs gp = g;
+  gp.C = work;
+  hipLaunchKernelGGL(k_gemm, dim3(gx, gy, nsplit), dim3(256), 0, s, gp, 1);
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(grid_for((long)M * N)), dim3(256), 0, s, g, (const float *)work, nsplit);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_gelu_bwd(long n, const float *Z, const float *dG, float *dZ, hipStream_t s) {
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(grid_for(n)), dim3(256), 0, s, n, Z, dG, dZ);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_colsum(long rows, int cols, const float *X, float *out, int accumulate, float *work,
+                                size_t work_floats, hipStream_t s) {
+  long nblk = 1024;
+  if ((size_t)nblk * cols > work_floats) nblk = (long)(work_floats / cols);
+  if (nblk < 1) return (int)hipErrorInvalidValue;
+  const long rpb = (rows + nblk - 1) / nblk;
+  nblk = (rows + rpb - 1) / rpb;
+  hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, X, work, rpb);
+  hipLaunchKernelGGL(k_colsum_final, dim3((cols + 255) / 256), dim3(256), 0, s, (int)nblk, cols, (const float *)work,
+                     out, accumulate);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_features(int N, int Kb, int ncell, const void *Fd, const void *Fm, const double *mom,
+                                  double fscale, float *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_features, dim3(grid_for((long)Kb * Kb * Kb * ncell)), dim3(256), 0, s, N, Kb, ncell,
+                     (const double2 *)Fd, (const double2 *)Fm, mom, fscale, X);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_targets(int N, int Kb, int ncell, const void *Qh, const double *mom, double tscale, float *T,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(k_net_targets, dim3(grid_for((long)Kb * Kb * Kb * ncell)), dim3(256), 0, s, N, Kb, ncell,
+                     (const double2 *)Qh, mom, tscale, T);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_scatter(int N, int Kb, int ncell, const float *Y, const double *mom, double tscale, void *Qh,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(k_net_scatter, dim3(grid_for((long)N * N * N * ncell)), dim3(256), 0, s, N, Kb, ncell, Y, mom,
+                     tscale, (double2 *)Qh);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_lift(long M, int ncell, int C, const float *X, const float *P, const float *E, const float *pb,
+                              float *H, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_lift, dim3(grid_for(M * ncell * C)), dim3(256), 0, s, M, ncell, C, X, P, E, pb, H);
+  return (int)hipGetLastError();
+}
+
+/* dE written; dP (4 x C) and dpb (C) written from per-mode partials (work >= M * 5C floats). */
+extern "C" int spbte_net_lift_bwd(long M, int ncell, int C, const float *X, const float *dH, float *dE, float *dP,
+                                  float *dpb, float *work, size_t work_floats, hipStream_t s) {
+  if ((size_t)M * 5 * C > work_floats) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_net_lift_bwd, dim3((unsigned)M), dim3(256), 0, s, M, ncell, C, X, dH, dE, work);
+  // column sums over modes of the [M][5C] partials: first 4C are dP, last C dpb
+  hipLaunchKernelGGL(k_colsum_final, dim3((5 * C + 255) / 256), dim3(256), 0, s, (int)M, 5 * C, (const float *)work,
+                     work + (size_t)M * 5 * C, 0);
+  hipError_t e = hipMemcpyAsync(dP, work + (size_t)M * 5 * C, sizeof(float) * 4 * C, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return (int)e;
+  e = hipMemcpyAsync(dpb, work + (size_t)M * 5 * C + 4 * C, sizeof(float) * C, hipMemcpyDeviceToDevice, s);
+  return (int)e;
+}
+
+extern "C" int spbte_net_head(long R, int C, const float *H, const float *O, const float *ob, float *Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_head, dim3(grid_for(R)), dim3(256), 0, s, R, C, H, O, ob, Y);
+  return (int)hipGetLastError();
+}
+
+/* dH written; dO (C x 2) and dob (2) from block partials (work >= 1024 * (2C + 2) + 2C + 2 floats). */
+extern "C" int spbte_net_head_bwd(long R, int C, const float *H, const float *O, const float *dY, float *dH, float *dO,
+                                  float *dob, float *work, size_t work_floats, hipStream_t s) {
+  if (C > 256 || 256 % C) return (int)hipErrorInvalidValue;
+  long nblk = 1024;
+  const size_t need = (size_t)nblk * (2 * C + 2) + 2 * C + 2;
+  if (need > work_floats) return (int)hipErrorInvalidValue;
+  const long rpb = (R + nblk - 1) / nblk;
+  nblk = (R + rpb - 1) / rpb;
+  hipLaunchKernelGGL(k_net_head_bwd, dim3((unsigned)nblk), dim3(256), 0, s, R, C, H, O, dY, dH, work, rpb);
+  float *fin = work + (size_t)1024 * (2 * C + 2);
+  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, s, (int)nblk, 2 * C + 2, (const float *)work, fin, 0);
+  hipError_t e = hipMemcpyAsync(dO, fin, sizeof(float) * 2 * C, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return (int)e;
+  e = hipMemcpyAsync(dob, fin + 2 * C, sizeof(float) * 2, hipMemcpyDeviceToDevice, s);
+  return (int)e;
+}
+
+extern "C" int spbte_net_loss(long M, int ncell, const float *Y, const float *T, float *dY, float *loss_b, float eps,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_net_loss, dim3(ncell), dim3(256), 0, s, M, ncell, Y, T, dY, loss_b, eps);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_adam(long n, float *p, const float *g, float *m1, float *m2, float lr, float b1, float b2,
+                              float eps, int step, float wd, hipStream_t s) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, s, n, p, g, m1, m2, lr, b1, b2, eps, c1, c2, wd);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_net_axpy(long n, float a, const float *x, float *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_f, dim3(grid_for(n)), dim3(256), 0, s, n, a, x, y);
+  return (int)hipGetLastError();
+}
+
+// Layout changes between the mode-major batch arrays ([m][b][w]) and the
+// sample-major dataset ([s][m][w]).  to_samples: dst[(idx[b] M + m) w + q] =
+// src[(m B + b) w + q]; otherwise the gather dst[(m B + b) w + q] = src[(idx[b] M + m) w + q].
+__global__ void __launch_bounds__(256) k_relayout(long M, int B, int w, const float *__restrict__ src,
+                                                  float *__restrict__ dst, int to_samples, const int *__restrict__ idx) {
+  const long total = M * B * w;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int q = (int)(e % w);
+    const long mb = e / w, m = mb / B;
+    const int b = (int)(mb - m * B);
+    const long se = ((long)idx[b] * M + m) * w + q;
+    if (to_samples) dst[se] = src[e];
+    else dst[e] = src[se];
+  }
+}
+
+extern "C" int spbte_net_relayout(long M, int nsamp, int width, const float *src, float *dst, int to_samples,
+                                  const int *idx, hipStream_t s) {
+  const long n = M * nsamp * width;
+  long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_relayout, dim3((unsigned)b), dim3(256), 0, s, M, nsamp, width, src, dst, to_samples, idx);
+  return (int)hipGetLastError();
+}

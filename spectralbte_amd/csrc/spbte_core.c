@@ -124,6 +124,9 @@ int spbte_op_synchronize(spbte_op *op) {
 hipStream_t spbte_op_stream(spbte_op *op) { return op->s; }
 const spbte_grid_dev *spbte_op_grid(spbte_op *op) { return &op->g; }
 unsigned spbte_op_generation(spbte_op *op) { return op->gen; }
+/* Qhat of the last chunk evaluated by spbte_compute_q (ncell * N^3 double2). */
+const void *spbte_op_qhat_buffer(spbte_op *op) { return op->d_Qh; }
+int spbte_op_chunk_cells(spbte_op *op);
 int spbte_op_n(spbte_op *op) { return op->N; }
 
 /* ------------------------------------------------------ transform matrices */
@@ -329,6 +332,8 @@ static int chunk_cells(const spbte_op *op) {
   if (c < 1) c = 1;
   return (int)c;
 }
+
+int spbte_op_chunk_cells(spbte_op *op) { return chunk_cells(op); }
 
 static int ensure_scratch(spbte_op *op, int ncell) {
   if (ncell <= op->cap) return SPBTE_OK;

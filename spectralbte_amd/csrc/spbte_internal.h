@@ -95,6 +95,34 @@ int spbte_launch_upwind2(int N, int nX, const double *f, double *fc, const doubl
 int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const double *y, double c, const double *z,
                          double *out, hipStream_t s);
 
+/* learned surrogate kernels (k_net.hip) */
+int spbte_net_gemm(const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0, long sc1,
+                   int M, int N, int K, int accumulate, const float *bias, float *G, float *work, size_t work_floats,
+                   hipStream_t s);
+int spbte_net_gelu_bwd(long n, const float *Z, const float *dG, float *dZ, hipStream_t s);
+int spbte_net_colsum(long rows, int cols, const float *X, float *out, int accumulate, float *work, size_t work_floats,
+                     hipStream_t s);
+int spbte_net_features(int N, int Kb, int ncell, const void *Fd, const void *Fm, const double *mom, double fscale,
+                       float *X, hipStream_t s);
+int spbte_net_targets(int N, int Kb, int ncell, const void *Qh, const double *mom, double tscale, float *T,
+                      hipStream_t s);
+int spbte_net_scatter(int N, int Kb, int ncell, const float *Y, const double *mom, double tscale, void *Qh,
+                      hipStream_t s);
+int spbte_net_lift(long M, int ncell, int C, const float *X, const float *P, const float *E, const float *pb, float *H,
+                   hipStream_t s);
+int spbte_net_lift_bwd(long M, int ncell, int C, const float *X, const float *dH, float *dE, float *dP, float *dpb,
+                       float *work, size_t work_floats, hipStream_t s);
+int spbte_net_head(long R, int C, const float *H, const float *O, const float *ob, float *Y, hipStream_t s);
+int spbte_net_head_bwd(long R, int C, const float *H, const float *O, const float *dY, float *dH, float *dO, float *dob,
+                       float *work, size_t work_floats, hipStream_t s);
+int spbte_net_loss(long M, int ncell, const float *Y, const float *T, float *dY, float *loss_b, float eps,
+                   hipStream_t s);
+int spbte_net_adam(long n, float *p, const float *g, float *m1, float *m2, float lr, float b1, float b2, float eps,
+                   int step, float wd, hipStream_t s);
+int spbte_net_axpy(long n, float a, const float *x, float *y, hipStream_t s);
+int spbte_net_relayout(long M, int nsamp, int width, const float *src, float *dst, int to_samples, const int *idx,
+                       hipStream_t s);
+
 /* host helpers shared by the reference-compatible modules (spbte_core.c) */
 int spbte_host_lu_factor(int n, double *A, int *pivot);
 void spbte_host_lu_solve(int n, const double *LU, const int *pivot, double *b);
@@ -104,6 +132,8 @@ hipStream_t spbte_op_stream(struct spbte_op *op);
 const spbte_grid_dev *spbte_op_grid(struct spbte_op *op);
 int spbte_op_get_weights_rows(struct spbte_op *op, double **rows);
 unsigned spbte_op_generation(struct spbte_op *op);
+const void *spbte_op_qhat_buffer(struct spbte_op *op);
+int spbte_op_chunk_cells(struct spbte_op *op);
 int spbte_op_n(struct spbte_op *op);
 int spbte_fail(int code, const char *fmt, ...);
 
