@@ -129,6 +129,11 @@ def lib():
             "spbte_net_forward_features": (I, [_vp, _fp, I, _fp]),
             "spbte_net_grad_features": (I, [_vp, _fp, _fp, I, _fp, _dp]),
             "spbte_net_predict": (I, [_vp, _vp, _vp, I, I]),
+            "spbte_net_collide": (I, [_vp, _vp, _vp, I, I, I, I, D, D, ip, _dp]),
+            "spbte_net_set_loss_floor": (I, [_vp, D]),
+            "spbte_run_set_surrogate": (I, [_vp, _vp, I, D, D]),
+            "spbte_run_surrogate_stats": (I, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                                              ctypes.POINTER(ctypes.c_long), _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -382,6 +387,17 @@ class Run:
         if sync:
             self.op.sync()
 
+    def set_surrogate(self, net, check_every=8, tol=1e-3, floor_rel=0.0):
+        _check(lib().spbte_run_set_surrogate(self.h, net.h if net is not None else None, int(check_every), float(tol),
+                                             float(floor_rel)))
+
+    def surrogate_stats(self):
+        """(cells, cells evaluated exactly, stages that fell back, largest sampled error)."""
+        a, b, c, d = ctypes.c_long(), ctypes.c_long(), ctypes.c_long(), ctypes.c_double()
+        _check(lib().spbte_run_surrogate_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                               ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
+
     def moments(self):
         """(ncell, 5): rho, u_x, u_y, u_z, T per cell."""
         out = np.empty((self.ncell, 5))
@@ -503,6 +519,16 @@ class Net:
 
     def predict_dev(self, d_f, d_Q, ncell, conserve=True):
         _check(lib().spbte_net_predict(self.h, d_f, d_Q, int(ncell), int(conserve)))
+
+    def collide_dev(self, d_f, d_Q, ncell, form="fp64", check_every=8, phase=0, tol=1e-3, floor_rel=0.0):
+        """Surrogate-assisted Q with the sampled exact check; returns (cells evaluated exactly, max sampled error)."""
+        ne, err = ctypes.c_int(), ctypes.c_double()
+        _check(lib().spbte_net_collide(self.h, d_f, d_Q, int(ncell), FORMS[form], int(check_every), int(phase),
+                                       float(tol), float(floor_rel), ctypes.byref(ne), ctypes.byref(err)))
+        return ne.value, err.value
+
+    def set_loss_floor(self, rms_floor):
+        _check(lib().spbte_net_set_loss_floor(self.h, float(rms_floor)))
 
     def predict(self, f, conserve=True):
         f = _c(f).reshape(-1, self.op.N ** 3)

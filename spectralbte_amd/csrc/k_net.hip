@@ -518,3 +518,49 @@ extern "C" int spbte_net_relayout(long M, int nsamp, int width, const float *src
   hipLaunchKernelGGL(k_relayout, dim3((unsigned)b), dim3(256), 0, s, M, nsamp, width, src, dst, to_samples, idx);
   return (int)hipGetLastError();
 }
+
+// Per-cell relative L2 difference |a_b - b_b| / |b_b| of ncell blocks of n doubles.
+__global__ void __launch_bounds__(256) k_cell_relerr(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                                     double *__restrict__ out) {
+  __shared__ double red[2][4];
+  const long c = blockIdx.x;
+  double num = 0.0, den = 0.0;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    const double x = a[c * n + e], y = b[c * n + e];
+    num += (x - y) * (x - y);
+    den += y * y;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    num += __shfl_down(num, off, 64);
+    den += __shfl_down(den, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = num; red[1][threadIdx.x >> 6] = den; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    num = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    den = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    out[c] = den > 0.0 ? sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
+  }
+}
+
+extern "C" int spbte_net_cell_relerr(long n, int ncell, const double *a, const double *b, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_relerr, dim3(ncell), dim3(256), 0, s, n, a, b, out);
+  return (int)hipGetLastError();
+}
+
+// Per-cell L2 norm of ncell blocks of n doubles.
+__global__ void __launch_bounds__(256) k_cell_norm(long n, const double *__restrict__ a, double *__restrict__ out) {
+  __shared__ double red[4];
+  const long c = blockIdx.x;
+  double s = 0.0;
+  for (long e = threadIdx.x; e < n; e += 256) s += a[c * n + e] * a[c * n + e];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[c] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+
+extern "C" int spbte_net_cell_norm(long n, int ncell, const double *a, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, n, a, out);
+  return (int)hipGetLastError();
+}

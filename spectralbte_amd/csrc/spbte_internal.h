@@ -120,6 +120,8 @@ int spbte_net_loss(long M, int ncell, const float *Y, const float *T, float *dY,
 int spbte_net_adam(long n, float *p, const float *g, float *m1, float *m2, float lr, float b1, float b2, float eps,
                    int step, float wd, hipStream_t s);
 int spbte_net_axpy(long n, float a, const float *x, float *y, hipStream_t s);
+int spbte_net_cell_relerr(long n, int ncell, const double *a, const double *b, double *out, hipStream_t s);
+int spbte_net_cell_norm(long n, int ncell, const double *a, double *out, hipStream_t s);
 int spbte_net_relayout(long M, int nsamp, int width, const float *src, float *dst, int to_samples, const int *idx,
                        hipStream_t s);
 

@@ -58,6 +58,7 @@ struct spbte_net {
   int N, Kb, C, R, L, maxpreserve;
   long M;
   double fscale, tscale;
+  float loss_eps; /* added to |t|^2 in each cell's loss denominator */
   /* parameters: one FP32 vector, with gradient and Adam moments */
   long npar, oP, opb, oE, oO, oob;
   layer_off lo[MAXL];
@@ -72,6 +73,9 @@ struct spbte_net {
   size_t work_floats;
   double *mom;
   void *Fd, *Fm, *xs, *Qh;
+  /* surrogate-assisted collisions: sampled cells */
+  int scap;
+  double *fs, *Qs, *Qn, *err;
   /* dataset (sample-major) */
   long ds_cap, ds_n;
   float *dsX, *dsT;
@@ -156,6 +160,9 @@ void spbte_net_destroy(spbte_net *n) {
   float *b[] = {n->par, n->grad, n->m1, n->m2, n->work, n->dsX, n->dsT};
   for (size_t i = 0; i < sizeof b / sizeof b[0]; i++)
     if (b[i]) hipFree(b[i]);
+  double *d[] = {n->fs, n->Qs, n->Qn, n->err};
+  for (size_t i = 0; i < sizeof d / sizeof d[0]; i++)
+    if (d[i]) hipFree(d[i]);
   free(n);
 }
 
@@ -178,6 +185,7 @@ int spbte_net_create(spbte_net **out, spbte_op *op, int Kb, int width, int rank,
   n->L = layers;
   n->maxpreserve = maxpreserve != 0;
   n->fscale = n->tscale = 1.0;
+  n->loss_eps = (float)(0.02 * (double)n->M); /* per-value RMS floor 0.1 in target units */
   const long M = n->M, C = width, r = rank;
   long o = 0;
   n->oP = o; o += 4 * C;
@@ -248,6 +256,13 @@ int spbte_net_set_scales(spbte_net *n, double fscale, double tscale) {
   if (!(fscale > 0) || !(tscale > 0)) return spbte_fail(SPBTE_ERR_ARG, "scales must be positive");
   n->fscale = fscale;
   n->tscale = tscale;
+  return SPBTE_OK;
+}
+
+int spbte_net_set_loss_floor(spbte_net *n, double rms_floor) {
+  if (!n || !(rms_floor >= 0)) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  n->loss_eps = (float)(rms_floor * rms_floor * 2.0 * (double)n->M);
+  if (n->loss_eps <= 0.0f) n->loss_eps = 1e-30f;
   return SPBTE_OK;
 }
 
@@ -408,7 +423,7 @@ static int batch_grad(spbte_net *n, const int *idx, int B, double *loss) {
   LCHK(spbte_net_relayout(n->M, B, 4, n->dsX, n->X, 0, n->d_idx, n->s));
   LCHK(spbte_net_relayout(n->M, B, 2, n->dsT, n->T, 0, n->d_idx, n->s));
   CHK(forward(n, B));
-  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, n->dY, n->loss, 1e-30f, n->s));
+  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, n->dY, n->loss, n->loss_eps, n->s));
   CHK(backward(n, B));
   if (loss) {
     float *lb = malloc(sizeof(float) * B);
@@ -480,7 +495,7 @@ int spbte_net_eval(spbte_net *n, const int *idx, int B, float *err) {
   LCHK(spbte_net_relayout(n->M, B, 4, n->dsX, n->X, 0, n->d_idx, n->s));
   LCHK(spbte_net_relayout(n->M, B, 2, n->dsT, n->T, 0, n->d_idx, n->s));
   CHK(forward(n, B));
-  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, NULL, n->loss, 1e-30f, n->s));
+  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, NULL, n->loss, n->loss_eps, n->s));
   HIPCHK(hipStreamSynchronize(n->s));
   HIPCHK(hipMemcpy(err, n->loss, sizeof(float) * B, hipMemcpyDeviceToHost));
   for (int b = 0; b < B; b++) err[b] = sqrtf(err[b]);
@@ -527,7 +542,7 @@ int spbte_net_grad_features(spbte_net *n, const float *X, const float *T, int B,
   HIPCHK(hipMemcpy(n->X, X, sizeof(float) * n->M * B * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(n->T, T, sizeof(float) * n->M * B * 2, hipMemcpyHostToDevice));
   CHK(forward(n, B));
-  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, n->dY, n->loss, 1e-30f, n->s));
+  LCHK(spbte_net_loss(n->M, B, n->Y, n->T, n->dY, n->loss, n->loss_eps, n->s));
   CHK(backward(n, B));
   HIPCHK(hipStreamSynchronize(n->s));
   if (grad) HIPCHK(hipMemcpy(grad, n->grad, sizeof(float) * n->npar, hipMemcpyDeviceToHost));
@@ -561,4 +576,81 @@ int spbte_net_layout(spbte_net *n, long *offsets) {
   offsets[k++] = n->oO;
   offsets[k++] = n->oob;
   return k;
+}
+
+/* Surrogate-assisted collision operator for ncell cells.  Every check_every-th
+ * cell (from `phase`) is evaluated exactly with the operator (form `form`);
+ * those cells keep the exact Q.  If the surrogate's relative L2 error on any
+ * of them exceeds tol, every cell falls back to the exact operator; otherwise
+ * the other cells take the surrogate Q.  Q is projected in every case.
+ * n_exact: cells evaluated exactly; max_err: largest sampled error. */
+int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, int form, int check_every, int phase,
+                      double tol, double floor_rel, int *n_exact, double *max_err) {
+  if (!n || !d_f || !d_Q || ncell < 1 || check_every < 1) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  const size_t n3 = (size_t)n->N * n->N * n->N;
+  const int flags = SPBTE_SELF | SPBTE_CONSERVE | (n->maxpreserve ? SPBTE_MAXPRESERVE : 0);
+  phase = ((phase % check_every) + check_every) % check_every;
+  const int k = phase < ncell ? (ncell - 1 - phase) / check_every + 1 : 0;
+  if (k > n->scap) {
+    HIPCHK(hipStreamSynchronize(n->s));
+    double **d[] = {&n->fs, &n->Qs, &n->Qn, &n->err};
+    for (size_t i = 0; i < 4; i++)
+      if (*d[i]) { hipFree(*d[i]); *d[i] = NULL; }
+    HIPCHK(hipMalloc((void **)&n->fs, sizeof(double) * n3 * k));
+    HIPCHK(hipMalloc((void **)&n->Qs, sizeof(double) * n3 * k));
+    HIPCHK(hipMalloc((void **)&n->Qn, sizeof(double) * n3 * k));
+    HIPCHK(hipMalloc((void **)&n->err, sizeof(double) * k));
+    n->scap = k;
+  }
+  CHK(spbte_net_predict(n, d_f, d_Q, ncell, 1));
+  double worst = 0.0;
+  if (k > 0) {
+    for (int j = 0; j < k; j++) {
+      const size_t c = (size_t)phase + (size_t)j * check_every;
+      HIPCHK(hipMemcpyAsync(n->fs + j * n3, d_f + c * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
+      HIPCHK(hipMemcpyAsync(n->Qn + j * n3, d_Q + c * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
+    }
+    CHK(spbte_compute_q(n->op, n->fs, NULL, n->Qs, k, form, flags));
+    /* err_j = |Qn_j - Qs_j| / max(|Qs_j|, floor_rel * max_i |Qs_i|) */
+    double *he = malloc(sizeof(double) * 3 * k);
+    if (!he) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+    int lrc = spbte_net_cell_relerr((long)n3, k, n->Qn, n->Qs, n->err, n->s);
+    hipError_t e = (hipError_t)lrc;
+    if (e == hipSuccess) e = hipMemcpyAsync(he, n->err, sizeof(double) * k, hipMemcpyDeviceToHost, n->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+    if (e == hipSuccess && floor_rel > 0) {
+      lrc = spbte_net_cell_norm((long)n3, k, n->Qs, n->err, n->s);
+      e = (hipError_t)lrc;
+      if (e == hipSuccess) e = hipMemcpyAsync(he + k, n->err, sizeof(double) * k, hipMemcpyDeviceToHost, n->s);
+      if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+      if (e == hipSuccess) e = (hipError_t)spbte_net_cell_norm((long)n3, k, n->Qn, n->err, n->s);
+      if (e == hipSuccess) e = hipMemcpyAsync(he + 2 * k, n->err, sizeof(double) * k, hipMemcpyDeviceToHost, n->s);
+      if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+      double qmax = 0.0;
+      for (int j = 0; j < k; j++) qmax = he[k + j] > qmax ? he[k + j] : qmax;
+      const double fl = floor_rel * qmax;
+      for (int j = 0; j < k && e == hipSuccess; j++) {
+        const double q = he[k + j];
+        if (q >= fl) continue;
+        const double dq = q > 0 ? he[j] * q : he[2 * k + j]; /* |Qn - Qs|; Qs = 0 gives |Qn| */
+        he[j] = fl > 0 ? dq / fl : he[j];
+      }
+    }
+    for (int j = 0; j < k && e == hipSuccess; j++)
+      if (!(he[j] <= worst)) worst = he[j]; /* NaN counts as a failure */
+    free(he);
+    if (e != hipSuccess) return spbte_fail(SPBTE_ERR_HIP, "error check failed: %s", hipGetErrorString(e));
+  }
+  if (max_err) *max_err = worst;
+  if (!(worst <= tol)) {
+    CHK(spbte_compute_q(n->op, d_f, NULL, d_Q, ncell, form, flags));
+    if (n_exact) *n_exact = ncell;
+    return SPBTE_OK;
+  }
+  for (int j = 0; j < k; j++) {
+    const size_t c = (size_t)phase + (size_t)j * check_every;
+    HIPCHK(hipMemcpyAsync(d_Q + c * n3, n->Qs + j * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
+  }
+  if (n_exact) *n_exact = k;
+  return SPBTE_OK;
 }

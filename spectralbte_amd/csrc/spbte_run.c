@@ -53,6 +53,12 @@ struct spbte_run {
   double *d_fl, *d_fr;           /* order-2 wall states */
   double *d_mom;
   int use_graph;
+  /* optional learned surrogate with its sampled check */
+  spbte_net *net;
+  int check_every, phase;
+  double tol, floor_rel;
+  long s_cells, s_exact, s_fallbacks;
+  double s_max_err;
   hipGraph_t graph;
   hipGraphExec_t exec;
   unsigned gen;
@@ -234,7 +240,40 @@ static int advect(spbte_run *r, double *src, double *dst) {
 
 /* ------------------------------------------------------------------- a step */
 static int collide(spbte_run *r, const double *d_in) {
-  return spbte_compute_q(r->op, d_in, NULL, r->d_Q, r->ncell, r->form, r->flags);
+  if (!r->net) return spbte_compute_q(r->op, d_in, NULL, r->d_Q, r->ncell, r->form, r->flags);
+  int n_exact = 0;
+  double err = 0.0;
+  CHK(spbte_net_collide(r->net, d_in, r->d_Q, r->ncell, r->form, r->check_every, r->phase, r->tol, r->floor_rel,
+                        &n_exact, &err));
+  r->phase = (r->phase + 1) % r->check_every;
+  r->s_cells += r->ncell;
+  r->s_exact += n_exact;
+  if (n_exact == r->ncell && r->check_every > 1) r->s_fallbacks++;
+  if (!(err <= r->s_max_err)) r->s_max_err = err;
+  return SPBTE_OK;
+}
+
+int spbte_run_set_surrogate(spbte_run *r, spbte_net *net, int check_every, double tol, double floor_rel) {
+  if (!r || check_every < 1 || !(tol > 0) || !(floor_rel >= 0)) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  HIPCHK(hipStreamSynchronize(spbte_op_stream(r->op)));
+  drop_graph(r);
+  r->net = net;
+  r->check_every = check_every;
+  r->tol = tol;
+  r->floor_rel = floor_rel;
+  r->phase = 0;
+  r->s_cells = r->s_exact = r->s_fallbacks = 0;
+  r->s_max_err = 0.0;
+  return SPBTE_OK;
+}
+
+int spbte_run_surrogate_stats(spbte_run *r, long *cells, long *exact, long *fallbacks, double *max_err) {
+  if (!r) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  if (cells) *cells = r->s_cells;
+  if (exact) *exact = r->s_exact;
+  if (fallbacks) *fallbacks = r->s_fallbacks;
+  if (max_err) *max_err = r->s_max_err;
+  return SPBTE_OK;
 }
 
 static int enqueue_step(spbte_run *r) {
@@ -312,7 +351,7 @@ int spbte_run_step(spbte_run *r, int nsteps) {
   if (!r || nsteps < 0) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
   hipStream_t s = spbte_op_stream(r->op);
   for (int t = 0; t < nsteps; t++) {
-    if (r->use_graph) {
+    if (r->use_graph && !r->net) { /* the surrogate's check synchronises with the host: no graph */
       if (!r->exec || r->gen != spbte_op_generation(r->op)) CHK(capture(r));
       HIPCHK(hipGraphLaunch(r->exec, s));
     } else {
