@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--no-ref", action="store_true", help="skip the FP64 comparisons")
     p.add_argument("--no-extra", action="store_true", help="skip the secondary workload")
     p.add_argument("--no-runs", action="store_true", help="skip the whole-step runs (shock, mixed ensemble)")
+    p.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     p.add_argument("--cpu-cells", type=int, default=0, help="problems in the CPU sample (0: about 15 s of work)")
     return p.parse_args()
 
@@ -186,6 +187,8 @@ def main():
     if not a.no_extra and not a.no_runs:
         extra["shock_run"] = shock_run(spb, a.form)
         extra["ensemble_mixed_run"] = mixed_ensemble_run(spb, a.form)
+    if not a.no_extra and not a.no_surrogate:
+        extra["surrogate"] = surrogate_run(spb, a.form)
 
     cpu = None
     if not a.no_cpu:
@@ -277,6 +280,65 @@ def mixed_ensemble_run(spb, form, ncell=512):
     return {"workload": f"0D-3V ensemble, {ncell} problems at N=24 + {ncell} at N=32, L_v=6, lambda=1, Heun dt=0.005, "
                         "ComputeQ_maxPreserve + projection, both grids' steps overlapped",
             "form": form, "ms_per_step": round(ms, 2), "q_per_s": round(q, 1), **per}
+
+
+def surrogate_run(spb, form, ncell=4096, train_cells=1024, steps=500):
+    """Learned surrogate on the N=32, L_v=8 hard-sphere grid: trained on the GPU
+    on exact Q of shock-profile cells, judged on held-out cells, then timed on a
+    4096-cell batch alone and inside the checked collision step (every 8th cell
+    exact, tolerance 1e-3, error floored at 0.1 of the largest |Q|)."""
+    from spectralbte_amd import surrogate as sg, workloads
+    N, Lv, Kb, C, r, L = 32, 8.0, 24, 64, 128, 4
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    net = spb.Net(op, Kb, C, r, L, maxpreserve=False, seed=1)
+    npar = net.npar
+    ftr = sg.shock_cells(N, Lv, train_cells, 1)
+    fte = sg.shock_cells(N, Lv, 256, 1001)
+    d_f = spb.DeviceArray.from_host(ftr[:256])
+    d_Q = spb.DeviceArray((256, N ** 3))
+    net.calibrate(d_f.ptr, d_Q.ptr, 256)
+    d_f.free(); d_Q.free()
+    t0 = time.perf_counter()
+    sg.build_dataset(spb, op, net, ftr)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = net.train(steps, 32, 1e-3)
+    t_train = time.perf_counter() - t0
+    rel, flo, qn = sg.judged_errors(spb, op, net, fte)
+    sig = qn >= 1e-2
+    f = workloads.shock_profile(N, Lv, ncell, mach=1.2, blend_cells=20.0 * ncell / 256)
+    t_pred = sg.time_predict(spb, op, net, f)
+    fl = sg.forward_flops(Kb ** 3, C, r, L) * ncell
+    d_f = spb.DeviceArray.from_host(f)
+    d_Q = spb.DeviceArray(f.shape)
+    op.autotune(ncell, form, False)
+    op.compute_q_dev(d_f.ptr, None, d_Q.ptr, ncell, form, spb.SELF | spb.CONSERVE)
+    op.sync()
+    t0 = time.perf_counter()
+    op.compute_q_dev(d_f.ptr, None, d_Q.ptr, ncell, form, spb.SELF | spb.CONSERVE)
+    op.sync()
+    t_exact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    n_exact, err = net.collide_dev(d_f.ptr, d_Q.ptr, ncell, form, check_every=8, phase=0, tol=1e-3, floor_rel=0.1)
+    t_coll = time.perf_counter() - t0
+    net.close()
+    op.close()
+    log(f"surrogate: train {steps} steps {t_train:.1f}s, loss {losses[-1]:.3g}; judged median rel err "
+        f"{np.median(rel[sig]):.3g}; predict {ncell / t_pred:.0f} Q/s ({fl / t_pred / 1e12:.1f} TFLOP/s); "
+        f"checked step: {n_exact} exact cells, max sampled err {err:.3g}")
+    return {"workload": f"learned surrogate, N=32, L_v=8, lambda=1, {Kb}^3 kept modes, width {C}, rank {r}, {L} layers "
+                        f"({npar} parameters), trained {steps} Adam steps of 32 "
+                        f"cells on {train_cells} shock-profile cells; timed on {ncell} cells (Mach 1.2 profile)",
+            "data_s": round(t_data, 2), "train_s": round(t_train, 2), "final_train_loss": float(losses[-1]),
+            "judged_rel_err_median_significant_cells": float(np.median(rel[sig])),
+            "judged_err_floor0.1_median": float(np.median(flo)), "judged_err_floor0.1_max": float(flo.max()),
+            "predict_q_per_s": round(ncell / t_pred, 1),
+            "predict_mfma_tflops": round(fl / t_pred / 1e12, 2), "mfma_peak_bf16_tflops": 2500.0,
+            "exact_q_per_s": round(ncell / t_exact, 1), "exact_form": form,
+            "checked_step_q_per_s": round(ncell / t_coll, 1), "checked_step_cells_exact": int(n_exact),
+            "checked_step_max_sampled_err": float(err), "check": "every 8th cell, tol 1e-3, floor 0.1 x max|Q|"}
 
 
 def cpu_baseline(wl, ncells):

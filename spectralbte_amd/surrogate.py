@@ -49,6 +49,24 @@ def judged_errors(spb, op, net, f, floor_rel=0.1, maxpreserve=False):
     return d / q, d / np.maximum(q, floor_rel * q.max()), q / q.max()
 
 
+def forward_flops(M, C, r, L):
+    """Multiply-add flops of one cell's forward pass (mixing and channel GEMMs)."""
+    return L * (2 * 2 * M * r * C + 2 * 2 * M * C * 2 * C)
+
+
+def time_predict(spb, op, net, f, reps=3):
+    """Seconds per surrogate evaluation of the cells f (inputs resident in HBM)."""
+    d_f = spb.DeviceArray.from_host(f)
+    d_Q = spb.DeviceArray(f.shape)
+    net.predict_dev(d_f.ptr, d_Q.ptr, len(f))
+    op.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.predict_dev(d_f.ptr, d_Q.ptr, len(f))
+    op.sync()
+    return (time.perf_counter() - t0) / reps
+
+
 def build_dataset(spb, op, net, f, chunk=256):
     """Append (features, exact targets) of the cells f to the network's dataset."""
     n = len(f)
