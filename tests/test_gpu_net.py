@@ -21,6 +21,7 @@ def _net_pair(spb, N, Kb, C, r, L, seed):
     eta = np.linspace(-3, 3, N)
     op = spb.Operator(N, 5.0, v, eta)
     net = spb.Net(op, Kb, width=C, rank=r, layers=L, seed=seed)
+    net.set_loss_floor(0.0)  # the restatement's loss is the plain relative error
     rng = np.random.default_rng(seed)
     off, n = no.layout(Kb ** 3, C, r, L)
     assert n == net.npar and net.offsets() == [off[k][0] for k in off]
