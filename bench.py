@@ -322,6 +322,7 @@ def surrogate_run(spb, form, ncell=4096, train_cells=1024, steps=500):
     t_exact = time.perf_counter() - t0
     t0 = time.perf_counter()
     n_exact, err = net.collide_dev(d_f.ptr, d_Q.ptr, ncell, form, check_every=8, phase=0, tol=1e-3, floor_rel=0.1)
+    op.sync()
     t_coll = time.perf_counter() - t0
     net.close()
     op.close()

@@ -644,6 +644,7 @@ int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, i
   if (max_err) *max_err = worst;
   if (!(worst <= tol)) {
     CHK(spbte_compute_q(n->op, d_f, NULL, d_Q, ncell, form, flags));
+    HIPCHK(hipStreamSynchronize(n->s));
     if (n_exact) *n_exact = ncell;
     return SPBTE_OK;
   }
@@ -651,6 +652,7 @@ int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, i
     const size_t c = (size_t)phase + (size_t)j * check_every;
     HIPCHK(hipMemcpyAsync(d_Q + c * n3, n->Qs + j * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
   }
+  HIPCHK(hipStreamSynchronize(n->s));
   if (n_exact) *n_exact = k;
   return SPBTE_OK;
 }
