@@ -12,6 +12,7 @@
 // sums with the epilogue.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdint.h>
 #include "spbte_internal.h"
 
 namespace {
@@ -30,56 +31,106 @@ __device__ __forceinline__ float gelu_grad(float x) {
 struct GemmArgs {
   const float *A, *B;
   float *C;           // output (or the split-K partials)
+  const float *add;   // addend with C's layout (may equal C), or null
   const float *bias;  // per column j, or null
   float *G;           // GELU(C) output (epilogue 1), or null
   long sa0, sa1, sb0, sb1, sc0, sc1;
   int M, N, K;
-  int accumulate;     // C += result instead of C = result
   int kslice;         // K per split (multiple of TK)
+  int vecA, vecB;     // 16-byte aligned rows along the contiguous dimension
 };
 
-// Stage a 64x32 tile of A (rows i0.., cols k0..) as sA[i][k] and a 32x64 tile
-// of B as sB[j][k], both k-contiguous, rounding to BF16.  Out-of-range
-// elements are zero.  Each of the 256 threads moves 8 elements of each.
-__device__ __forceinline__ void stage(const GemmArgs &g, __bf16 (*sA)[KP], __bf16 (*sB)[KP], int i0, int j0, int k0,
-                                      int kend) {
+// Register image of one thread's share of a K-tile: 8 values of A and 8 of B.
+struct Stage {
+  float a[8], b[8];
+};
+
+// Thread t covers, for A: row-major (sa1 == 1) row t/4, k (t%4)*8 .. +8;
+// column-major row (t%8)*8 .. +8, k t/8.  For B: k-contiguous (sb0 == 1)
+// column t/4, k (t%4)*8 .. +8; row-major column (t%8)*8 .. +8, k t/8.
+__device__ __forceinline__ void load_stage(const GemmArgs &g, Stage &r, int i0, int j0, int k0, int kend) {
   const int t = threadIdx.x;
-  if (g.sa1 == 1) {  // A row-major: thread reads 8 consecutive k of one row
-    const int i = t >> 2, kk = (t & 3) * 8;
+  if (g.sa1 == 1) {
+    const int gi = i0 + (t >> 2), gk = k0 + (t & 3) * 8;
+    const float *p = g.A + gi * g.sa0 + gk;
+    if (g.vecA && gi < g.M && gk + 8 <= kend) {
+      const float4 u = *(const float4 *)p, v = *(const float4 *)(p + 4);
+      r.a[0] = u.x; r.a[1] = u.y; r.a[2] = u.z; r.a[3] = u.w; r.a[4] = v.x; r.a[5] = v.y; r.a[6] = v.z; r.a[7] = v.w;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int gi = i0 + i, gk = k0 + kk + e;
-      sA[i][kk + e] = (__bf16)((gi < g.M && gk < kend) ? g.A[gi * g.sa0 + gk] : 0.0f);
+      for (int e = 0; e < 8; ++e) r.a[e] = (gi < g.M && gk + e < kend) ? p[e] : 0.0f;
     }
-  } else {           // A column-major: thread reads 8 consecutive i of one k
-    const int kk = t >> 3, i = (t & 7) * 8;
+  } else {
+    const int gi = i0 + (t & 7) * 8, gk = k0 + (t >> 3);
+    const float *p = g.A + gi + gk * g.sa1;
+    if (g.vecA && gi + 8 <= g.M && gk < kend) {
+      const float4 u = *(const float4 *)p, v = *(const float4 *)(p + 4);
+      r.a[0] = u.x; r.a[1] = u.y; r.a[2] = u.z; r.a[3] = u.w; r.a[4] = v.x; r.a[5] = v.y; r.a[6] = v.z; r.a[7] = v.w;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int gi = i0 + i + e, gk = k0 + kk;
-      sA[i + e][kk] = (__bf16)((gi < g.M && gk < kend) ? g.A[gi * g.sa0 + gk * g.sa1] : 0.0f);
+      for (int e = 0; e < 8; ++e) r.a[e] = (gi + e < g.M && gk < kend) ? p[e] : 0.0f;
     }
   }
-  if (g.sb0 == 1) {  // B stored with k contiguous (column-major): 8 consecutive k of one j
-    const int j = t >> 2, kk = (t & 3) * 8;
+  if (g.sb0 == 1) {
+    const int gj = j0 + (t >> 2), gk = k0 + (t & 3) * 8;
+    const float *p = g.B + gk + gj * g.sb1;
+    if (g.vecB && gj < g.N && gk + 8 <= kend) {
+      const float4 u = *(const float4 *)p, v = *(const float4 *)(p + 4);
+      r.b[0] = u.x; r.b[1] = u.y; r.b[2] = u.z; r.b[3] = u.w; r.b[4] = v.x; r.b[5] = v.y; r.b[6] = v.z; r.b[7] = v.w;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int gj = j0 + j, gk = k0 + kk + e;
-      sB[j][kk + e] = (__bf16)((gj < g.N && gk < kend) ? g.B[gk + gj * g.sb1] : 0.0f);
+      for (int e = 0; e < 8; ++e) r.b[e] = (gj < g.N && gk + e < kend) ? p[e] : 0.0f;
     }
-  } else {           // B row-major: 8 consecutive j of one k
-    const int kk = t >> 3, j = (t & 7) * 8;
+  } else {
+    const int gj = j0 + (t & 7) * 8, gk = k0 + (t >> 3);
+    const float *p = g.B + gk * g.sb0 + gj;
+    if (g.vecB && gj + 8 <= g.N && gk < kend) {
+      const float4 u = *(const float4 *)p, v = *(const float4 *)(p + 4);
+      r.b[0] = u.x; r.b[1] = u.y; r.b[2] = u.z; r.b[3] = u.w; r.b[4] = v.x; r.b[5] = v.y; r.b[6] = v.z; r.b[7] = v.w;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int gj = j0 + j + e, gk = k0 + kk;
-      sB[j + e][kk] = (__bf16)((gj < g.N && gk < kend) ? g.B[gk * g.sb0 + gj * g.sb1] : 0.0f);
+      for (int e = 0; e < 8; ++e) r.b[e] = (gj + e < g.N && gk < kend) ? p[e] : 0.0f;
     }
   }
 }
 
-// epi: 0 plain (C = / += acc), 1 bias + GELU copy.  split: write partials.
+// Round to BF16 and write into the k-contiguous LDS tiles sA[i][k], sB[j][k].
+__device__ __forceinline__ void store_stage(const GemmArgs &g, const Stage &r, __bf16 (*sA)[KP], __bf16 (*sB)[KP]) {
+  const int t = threadIdx.x;
+  if (g.sa1 == 1) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)r.a[e];
+    *(bf16x8 *)&sA[t >> 2][(t & 3) * 8] = v;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) sA[(t & 7) * 8 + e][t >> 3] = (__bf16)r.a[e];
+  }
+  if (g.sb0 == 1) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)r.b[e];
+    *(bf16x8 *)&sB[t >> 2][(t & 3) * 8] = v;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) sB[(t & 7) * 8 + e][t >> 3] = (__bf16)r.b[e];
+  }
+}
+
+__device__ __forceinline__ void epilogue(const GemmArgs &g, int i, int j, float v) {
+  const long o = i * g.sc0 + j * g.sc1;
+  float out = g.add ? g.add[o] + v : v;
+  if (g.bias) out += g.bias[j];
+  g.C[o] = out;
+  if (g.G) g.G[o] = gelu(out);
+}
+
+// The K-loop keeps the next tile's global loads in registers while the MFMAs
+// consume the current LDS buffer; two LDS buffers, one barrier per K-tile.
+// split: write FP32 partials [z][i][j] instead of the epilogue.
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g, int split) {
-  __shared__ __attribute__((aligned(16))) __bf16 sA[TM][KP];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[TN][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][TM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][TN][KP];
   const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   const int kbeg = blockIdx.z * g.kslice;
   const int kend = min(g.K, kbeg + g.kslice);
@@ -90,21 +141,28 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g, int split) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  Stage st;
+  load_stage(g, st, i0, j0, kbeg, kend);
+  store_stage(g, st, sA[0], sB[0]);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = kbeg; k0 < kend; k0 += TK) {
-    __syncthreads();
-    stage(g, sA, sB, i0, j0, k0, kend);
-    __syncthreads();
+    const bool more = k0 + TK < kend;
+    if (more) load_stage(g, st, i0, j0, k0 + TK, kend);
     // lane l: row/col (l % 16), k group 8 (l / 16) .. + 8
     const int r = lane & 15, kg = (lane >> 4) * 8;
     bf16x8 av[2], bv[2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = *(const bf16x8 *)&sA[wi + 16 * a + r][kg];
+    for (int a = 0; a < 2; ++a) av[a] = *(const bf16x8 *)&sA[cur][wi + 16 * a + r][kg];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x8 *)&sB[wj + 16 * b + r][kg];
+    for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x8 *)&sB[cur][wj + 16 * b + r][kg];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    if (more) store_stage(g, st, sA[cur ^ 1], sB[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
   }
   // D layout: lane l holds rows 4 (l / 16) + q, column l % 16 of each 16x16 tile
 #pragma unroll
@@ -116,15 +174,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g, int split) {
         const int i = i0 + wi + 16 * a + 4 * (lane >> 4) + q, j = j0 + wj + 16 * b + (lane & 15);
         if (i >= g.M || j >= g.N) continue;
         const float v = acc[a][b][q];
-        if (split) {
-          g.C[((long)blockIdx.z * g.M + i) * g.N + j] = v;  // partials: [z][i][j]
-          continue;
-        }
-        float *c = g.C + i * g.sc0 + j * g.sc1;
-        float o = g.accumulate ? *c + v : v;
-        if (g.bias) o += g.bias[j];
-        *c = o;
-        if (g.G) g.G[i * g.sc0 + j * g.sc1] = gelu(o);
+        if (split) g.C[((long)blockIdx.z * g.M + i) * g.N + j] = v;
+        else epilogue(g, i, j, v);
       }
 }
 
@@ -134,11 +185,7 @@ __global__ void __launch_bounds__(256) k_splitk_reduce(GemmArgs g, const float *
     const int i = (int)(e / g.N), j = (int)(e - (long)i * g.N);
     float v = 0.f;
     for (int z = 0; z < nsplit; ++z) v += part[(long)z * n + e];
-    float *c = g.C + i * g.sc0 + j * g.sc1;
-    float o = g.accumulate ? *c + v : v;
-    if (g.bias) o += g.bias[j];
-    *c = o;
-    if (g.G) g.G[i * g.sc0 + j * g.sc1] = gelu(o);
+    epilogue(g, i, j, v);
   }
 }
 
@@ -150,21 +197,34 @@ __global__ void __launch_bounds__(256) k_gelu_bwd(long n, const float *__restric
 }
 
 // Column sums of X (rows x cols, row-major) into partial[blk][col]; second stage sums blocks.
+// A workgroup covers 64 columns with 4 row groups (256 threads) and reduces
+// the groups in LDS; grid (column chunks, row blocks).
 __global__ void __launch_bounds__(256) k_colsum_part(long rows, int cols, const float *__restrict__ X, float *__restrict__ part,
                                                      long rows_per_blk) {
-  const long r0 = (long)blockIdx.x * rows_per_blk, r1 = min(rows, r0 + rows_per_blk);
-  for (int c = threadIdx.x; c < cols; c += 256) {
-    float s = 0.f;
-    for (long r = r0; r < r1; ++r) s += X[r * cols + c];
-    part[(long)blockIdx.x * cols + c] = s;
-  }
+  __shared__ float red[4][64];
+  const int lc = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lc;
+  const long r0 = (long)blockIdx.y * rows_per_blk, r1 = min(rows, r0 + rows_per_blk);
+  float s = 0.f;
+  if (c < cols)
+    for (long r = r0 + rg; r < r1; r += 4) s += X[r * cols + c];
+  red[rg][lc] = s;
+  __syncthreads();
+  if (rg == 0 && c < cols) part[(long)blockIdx.y * cols + c] = red[0][lc] + red[1][lc] + red[2][lc] + red[3][lc];
 }
 __global__ void __launch_bounds__(256) k_colsum_final(int nblk, int cols, const float *__restrict__ part, float *out,
                                                       int accumulate) {
-  for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) {
-    float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += part[(long)b * cols + c];
-    out[c] = accumulate ? out[c] + s : s;
+  __shared__ float red[4][64];
+  const int lc = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lc;
+  float s = 0.f;
+  if (c < cols)
+    for (int b = rg; b < nblk; b += 4) s += part[(long)b * cols + c];
+  red[rg][lc] = s;
+  __syncthreads();
+  if (rg == 0 && c < cols) {
+    const float t = red[0][lc] + red[1][lc] + red[2][lc] + red[3][lc];
+    out[c] = accumulate ? out[c] + t : t;
   }
 }
 
@@ -242,30 +302,36 @@ __global__ void __launch_bounds__(256) k_net_lift(long M, int ncell, int C, cons
   }
 }
 
-// Lift backward: dP[q][c] and dpb[c] need sums over (m, b); dE[m][c] = sum_b dH[m][b][c].
-// One workgroup per mode m: dE directly, and per-mode partials of dP, dpb.
+// Lift backward: dE[m][c] = sum_b dH[m][b][c]; dP[q][c] and dpb[c] need sums
+// over (m, b): one workgroup per LIFT_MB consecutive modes writes dE for them
+// and one row of partial sums [5C] (4 rows of dP, then dpb).
+constexpr int LIFT_MB = 64;
 __global__ void __launch_bounds__(256) k_net_lift_bwd(long M, int ncell, int C, const float *__restrict__ X,
                                                       const float *__restrict__ dH, float *__restrict__ dE,
-                                                      float *__restrict__ part /* [M][5*C] */) {
-  const long m = blockIdx.x;
+                                                      float *__restrict__ part /* [ceil(M / LIFT_MB)][5C] */) {
+  const long m0 = (long)blockIdx.x * LIFT_MB, m1 = min(M, m0 + LIFT_MB);
   for (int c = threadIdx.x; c < C; c += 256) {
-    float se = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    for (int b = 0; b < ncell; ++b) {
-      const float d = dH[(m * ncell + b) * C + c];
-      const float *x = X + (m * ncell + b) * 4;
-      se += d;
-      s0 += d * x[0];
-      s1 += d * x[1];
-      s2 += d * x[2];
-      s3 += d * x[3];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, sb = 0.f;
+    for (long m = m0; m < m1; ++m) {
+      float se = 0.f;
+      for (int b = 0; b < ncell; ++b) {
+        const float d = dH[(m * ncell + b) * C + c];
+        const float *x = X + (m * ncell + b) * 4;
+        se += d;
+        s0 += d * x[0];
+        s1 += d * x[1];
+        s2 += d * x[2];
+        s3 += d * x[3];
+      }
+      dE[m * C + c] = se;
+      sb += se;
     }
-    dE[m * C + c] = se;
-    float *p = part + m * 5 * C;
+    float *p = part + (long)blockIdx.x * 5 * C;
     p[c] = s0;
     p[C + c] = s1;
     p[2 * C + c] = s2;
     p[3 * C + c] = s3;
-    p[4 * C + c] = se;
+    p[4 * C + c] = sb;
   }
 }
 
@@ -363,11 +429,14 @@ inline unsigned grid_for(long n) {
 
 // ---- launchers ---------------------------------------------------------------
 extern "C" int spbte_net_gemm(const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0,
-                              long sc1, int M, int N, int K, int accumulate, const float *bias, float *G, float *work,
+                              long sc1, int M, int N, int K, const float *add, const float *bias, float *G, float *work,
                               size_t work_floats, hipStream_t s) {
   if (M <= 0 || N <= 0) return 0;
   if (!((sa1 == 1) || (sa0 == 1)) || !((sb0 == 1) || (sb1 == 1))) return (int)hipErrorInvalidValue;
-  GemmArgs g = {A, B, C, bias, G, sa0, sa1, sb0, sb1, sc0, sc1, M, N, K, accumulate, K};
+  GemmArgs g = {A, B, C, add, bias, G, sa0, sa1, sb0, sb1, sc0, sc1, M, N, K, K, 0, 0};
+  const long lda = sa1 == 1 ? sa0 : sa1, ldb = sb0 == 1 ? sb1 : sb0;
+  g.vecA = ((uintptr_t)A % 16 == 0) && (lda % 4 == 0);
+  g.vecB = ((uintptr_t)B % 16 == 0) && (ldb % 4 == 0);
   const unsigned gx = (N + TN - 1) / TN, gy = (M + TM - 1) / TM;
   // split K when the output tile grid alone cannot fill the chip
   int nsplit = 1;
@@ -405,9 +474,9 @@ extern "C" int spbte_net_colsum(long rows, int cols, const float *X, float *out,
   if (nblk < 1) return (int)hipErrorInvalidValue;
   const long rpb = (rows + nblk - 1) / nblk;
   nblk = (rows + rpb - 1) / rpb;
-  hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, X, work, rpb);
-  hipLaunchKernelGGL(k_colsum_final, dim3((cols + 255) / 256), dim3(256), 0, s, (int)nblk, cols, (const float *)work,
-                     out, accumulate);
+  hipLaunchKernelGGL(k_colsum_part, dim3((cols + 63) / 64, (unsigned)nblk), dim3(256), 0, s, rows, cols, X, work, rpb);
+  hipLaunchKernelGGL(k_colsum_final, dim3((cols + 63) / 64), dim3(256), 0, s, (int)nblk, cols, (const float *)work, out,
+                     accumulate);
   return (int)hipGetLastError();
 }
 
@@ -438,17 +507,19 @@ extern "C" int spbte_net_lift(long M, int ncell, int C, const float *X, const fl
   return (int)hipGetLastError();
 }
 
-/* dE written; dP (4 x C) and dpb (C) written from per-mode partials (work >= M * 5C floats). */
+/* dE written; dP (4 x C) and dpb (C) from per-block partials (work >= (nblk + 1) * 5C floats). */
 extern "C" int spbte_net_lift_bwd(long M, int ncell, int C, const float *X, const float *dH, float *dE, float *dP,
                                   float *dpb, float *work, size_t work_floats, hipStream_t s) {
-  if ((size_t)M * 5 * C > work_floats) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_net_lift_bwd, dim3((unsigned)M), dim3(256), 0, s, M, ncell, C, X, dH, dE, work);
-  // column sums over modes of the [M][5C] partials: first 4C are dP, last C dpb
-  hipLaunchKernelGGL(k_colsum_final, dim3((5 * C + 255) / 256), dim3(256), 0, s, (int)M, 5 * C, (const float *)work,
-                     work + (size_t)M * 5 * C, 0);
-  hipError_t e = hipMemcpyAsync(dP, work + (size_t)M * 5 * C, sizeof(float) * 4 * C, hipMemcpyDeviceToDevice, s);
+  const long nblk = (M + LIFT_MB - 1) / LIFT_MB;
+  if ((size_t)(nblk + 1) * 5 * C > work_floats) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_net_lift_bwd, dim3((unsigned)nblk), dim3(256), 0, s, M, ncell, C, X, dH, dE, work);
+  // column sums of the [nblk][5C] partials: first 4C are dP, last C dpb
+  float *fin = work + (size_t)nblk * 5 * C;
+  hipLaunchKernelGGL(k_colsum_final, dim3((5 * C + 63) / 64), dim3(256), 0, s, (int)nblk, 5 * C, (const float *)work,
+                     fin, 0);
+  hipError_t e = hipMemcpyAsync(dP, fin, sizeof(float) * 4 * C, hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return (int)e;
-  e = hipMemcpyAsync(dpb, work + (size_t)M * 5 * C + 4 * C, sizeof(float) * C, hipMemcpyDeviceToDevice, s);
+  e = hipMemcpyAsync(dpb, fin + 4 * C, sizeof(float) * C, hipMemcpyDeviceToDevice, s);
   return (int)e;
 }
 
@@ -468,7 +539,8 @@ extern "C" int spbte_net_head_bwd(long R, int C, const float *H, const float *O,
   nblk = (R + rpb - 1) / rpb;
   hipLaunchKernelGGL(k_net_head_bwd, dim3((unsigned)nblk), dim3(256), 0, s, R, C, H, O, dY, dH, work, rpb);
   float *fin = work + (size_t)1024 * (2 * C + 2);
-  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, s, (int)nblk, 2 * C + 2, (const float *)work, fin, 0);
+  hipLaunchKernelGGL(k_colsum_final, dim3((2 * C + 2 + 63) / 64), dim3(256), 0, s, (int)nblk, 2 * C + 2,
+                     (const float *)work, fin, 0);
   hipError_t e = hipMemcpyAsync(dO, fin, sizeof(float) * 2 * C, hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return (int)e;
   e = hipMemcpyAsync(dob, fin + 2 * C, sizeof(float) * 2, hipMemcpyDeviceToDevice, s);

@@ -97,7 +97,7 @@ int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const do
 
 /* learned surrogate kernels (k_net.hip) */
 int spbte_net_gemm(const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0, long sc1,
-                   int M, int N, int K, int accumulate, const float *bias, float *G, float *work, size_t work_floats,
+                   int M, int N, int K, const float *add, const float *bias, float *G, float *work, size_t work_floats,
                    hipStream_t s);
 int spbte_net_gelu_bwd(long n, const float *Z, const float *dG, float *dZ, hipStream_t s);
 int spbte_net_colsum(long rows, int cols, const float *X, float *out, int accumulate, float *work, size_t work_floats,

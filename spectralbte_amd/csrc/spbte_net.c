@@ -276,9 +276,10 @@ int spbte_net_get_scales(spbte_net *n, double *fscale, double *tscale) {
 #define P_(off) (n->par + (off))
 #define G_(off) (n->grad + (off))
 
+/* C = op(A) op(B) (+ add) (+ bias), G = GELU(C) if G; add has C's layout. */
 static int gemm(spbte_net *n, const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0,
-                long sc1, int M, int N, int K, int acc, const float *bias, float *G) {
-  LCHK(spbte_net_gemm(A, sa0, sa1, B, sb0, sb1, C, sc0, sc1, M, N, K, acc, bias, G, n->work, n->work_floats, n->s));
+                long sc1, int M, int N, int K, const float *add, const float *bias, float *G) {
+  LCHK(spbte_net_gemm(A, sa0, sa1, B, sb0, sb1, C, sc0, sc1, M, N, K, add, bias, G, n->work, n->work_floats, n->s));
   return SPBTE_OK;
 }
 
@@ -289,17 +290,15 @@ static int forward(spbte_net *n, int B) {
   for (int l = 0; l < n->L; l++) {
     const layer_off *q = &n->lo[l];
     /* S = V^T H  (r x BC) */
-    CHK(gemm(n, P_(q->V), 1, n->R, n->H[l], BC, 1, n->S[l], BC, 1, n->R, (int)BC, (int)M, 0, NULL, NULL));
+    CHK(gemm(n, P_(q->V), 1, n->R, n->H[l], BC, 1, n->S[l], BC, 1, n->R, (int)BC, (int)M, NULL, NULL, NULL));
     /* Hm = H + U S */
-    HIPCHK(hipMemcpyAsync(n->Hm[l], n->H[l], sizeof(float) * R * C, hipMemcpyDeviceToDevice, n->s));
-    CHK(gemm(n, P_(q->U), n->R, 1, n->S[l], BC, 1, n->Hm[l], BC, 1, (int)M, (int)BC, n->R, 1, NULL, NULL));
+    CHK(gemm(n, P_(q->U), n->R, 1, n->S[l], BC, 1, n->Hm[l], BC, 1, (int)M, (int)BC, n->R, n->H[l], NULL, NULL));
     /* Z = Hm W1 + b1, G = GELU(Z) */
-    CHK(gemm(n, n->Hm[l], C, 1, P_(q->W1), 2 * C, 1, n->Z[l], 2 * C, 1, (int)R, (int)(2 * C), (int)C, 0, P_(q->b1),
+    CHK(gemm(n, n->Hm[l], C, 1, P_(q->W1), 2 * C, 1, n->Z[l], 2 * C, 1, (int)R, (int)(2 * C), (int)C, NULL, P_(q->b1),
              n->G[l]));
     /* H' = Hm + G W2 + b2 */
-    HIPCHK(hipMemcpyAsync(n->H[l + 1], n->Hm[l], sizeof(float) * R * C, hipMemcpyDeviceToDevice, n->s));
-    CHK(gemm(n, n->G[l], 2 * C, 1, P_(q->W2), C, 1, n->H[l + 1], C, 1, (int)R, (int)C, (int)(2 * C), 1, P_(q->b2),
-             NULL));
+    CHK(gemm(n, n->G[l], 2 * C, 1, P_(q->W2), C, 1, n->H[l + 1], C, 1, (int)R, (int)C, (int)(2 * C), n->Hm[l],
+             P_(q->b2), NULL));
   }
   LCHK(spbte_net_head(R, (int)C, n->H[n->L], P_(n->oO), P_(n->oob), n->Y, n->s));
   return SPBTE_OK;
@@ -313,21 +312,19 @@ static int backward(spbte_net *n, int B) {
   for (int l = n->L - 1; l >= 0; l--) {
     const layer_off *q = &n->lo[l];
     /* channel MLP: dG = dH W2^T, dW2 = G^T dH, db2 = colsum dH */
-    CHK(gemm(n, n->dH, C, 1, P_(q->W2), 1, C, n->dG, 2 * C, 1, (int)R, (int)(2 * C), (int)C, 0, NULL, NULL));
-    CHK(gemm(n, n->G[l], 1, 2 * C, n->dH, C, 1, G_(q->W2), C, 1, (int)(2 * C), (int)C, (int)R, 0, NULL, NULL));
+    CHK(gemm(n, n->dH, C, 1, P_(q->W2), 1, C, n->dG, 2 * C, 1, (int)R, (int)(2 * C), (int)C, NULL, NULL, NULL));
+    CHK(gemm(n, n->G[l], 1, 2 * C, n->dH, C, 1, G_(q->W2), C, 1, (int)(2 * C), (int)C, (int)R, NULL, NULL, NULL));
     LCHK(spbte_net_colsum(R, (int)C, n->dH, G_(q->b2), 0, n->work, n->work_floats, n->s));
     LCHK(spbte_net_gelu_bwd(R * 2 * C, n->Z[l], n->dG, n->dZ, n->s));
-    CHK(gemm(n, n->Hm[l], 1, C, n->dZ, 2 * C, 1, G_(q->W1), 2 * C, 1, (int)C, (int)(2 * C), (int)R, 0, NULL, NULL));
+    CHK(gemm(n, n->Hm[l], 1, C, n->dZ, 2 * C, 1, G_(q->W1), 2 * C, 1, (int)C, (int)(2 * C), (int)R, NULL, NULL, NULL));
     LCHK(spbte_net_colsum(R, (int)(2 * C), n->dZ, G_(q->b1), 0, n->work, n->work_floats, n->s));
     /* dHm = dH + dZ W1^T */
-    HIPCHK(hipMemcpyAsync(n->dHm, n->dH, sizeof(float) * R * C, hipMemcpyDeviceToDevice, n->s));
-    CHK(gemm(n, n->dZ, 2 * C, 1, P_(q->W1), 1, 2 * C, n->dHm, C, 1, (int)R, (int)C, (int)(2 * C), 1, NULL, NULL));
+    CHK(gemm(n, n->dZ, 2 * C, 1, P_(q->W1), 1, 2 * C, n->dHm, C, 1, (int)R, (int)C, (int)(2 * C), n->dH, NULL, NULL));
     /* mode mixing: dU = dHm S^T, dS = U^T dHm, dH = dHm + V dS, dV = H dS^T */
-    CHK(gemm(n, n->dHm, BC, 1, n->S[l], 1, BC, G_(q->U), n->R, 1, (int)M, n->R, (int)BC, 0, NULL, NULL));
-    CHK(gemm(n, P_(q->U), 1, n->R, n->dHm, BC, 1, n->dS, BC, 1, n->R, (int)BC, (int)M, 0, NULL, NULL));
-    HIPCHK(hipMemcpyAsync(n->dH, n->dHm, sizeof(float) * R * C, hipMemcpyDeviceToDevice, n->s));
-    CHK(gemm(n, P_(q->V), n->R, 1, n->dS, BC, 1, n->dH, BC, 1, (int)M, (int)BC, n->R, 1, NULL, NULL));
-    CHK(gemm(n, n->H[l], BC, 1, n->dS, 1, BC, G_(q->V), n->R, 1, (int)M, n->R, (int)BC, 0, NULL, NULL));
+    CHK(gemm(n, n->dHm, BC, 1, n->S[l], 1, BC, G_(q->U), n->R, 1, (int)M, n->R, (int)BC, NULL, NULL, NULL));
+    CHK(gemm(n, P_(q->U), 1, n->R, n->dHm, BC, 1, n->dS, BC, 1, n->R, (int)BC, (int)M, NULL, NULL, NULL));
+    CHK(gemm(n, P_(q->V), n->R, 1, n->dS, BC, 1, n->dH, BC, 1, (int)M, (int)BC, n->R, n->dHm, NULL, NULL));
+    CHK(gemm(n, n->H[l], BC, 1, n->dS, 1, BC, G_(q->V), n->R, 1, (int)M, n->R, (int)BC, NULL, NULL, NULL));
   }
   LCHK(spbte_net_lift_bwd(M, B, (int)C, n->X, n->dH, G_(n->oE), G_(n->oP), G_(n->opb), n->work, n->work_floats, n->s));
   return SPBTE_OK;
