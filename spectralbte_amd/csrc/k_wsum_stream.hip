@@ -44,6 +44,16 @@ struct SGeom {
   // window refill of every step of a body reads at (base - j + BS) with j the
   // step's constant offset in the body, and never wraps
   static constexpr int LW = N + BS;
+  // Line storage.  When every lane's window start is a multiple of 4 (TR and
+  // N/2 multiples of 4), entry m is stored at (m mod 4) * LW/4 + m / 4: the
+  // refills of the lanes of one z-line (entries 4 apart) then fall on
+  // consecutive 16-byte slots instead of one slot in four of the banks, and a
+  // refill offset stays a compile-time constant per body step.  Otherwise
+  // entries are stored in order.
+  static constexpr bool TL = (TR % 4 == 0) && ((N / 2) % 4 == 0);
+  static constexpr int LS = LW / 4;
+  static_assert(!TL || LW % 4 == 0, "transposed lines need LW % 4 == 0");
+  __host__ __device__ static constexpr int pos(int m) { return TL ? (m & 3) * LS + (m >> 2) : m; }
   static_assert(N % BS == 0, "cz body must divide N");
   static_assert(N % TR == 0, "TR must divide N");
   static_assert((N * N * N) % RWG == 0, "workgroups must tile the output rows");
@@ -140,7 +150,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       int sx = (rx0 + q) % N + N / 2 - cx;
       sx = sx < 0 ? sx + N : (sx > N - 1 ? sx - N : sx);
       const int z = m < BS ? m - BS + N : m - BS;
-      sF[e] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
+      sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
     }
     for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
       const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
@@ -164,7 +174,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
         for (int b = 0; b < TB; ++b)
 #pragma unroll
-          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][BS + ((k0 + t) % N)];
+          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][Gm::pos(BS + ((k0 + t) % N))];
       const int c4base = (cx * N + cy) * (N / 4);
       const double2 *gline = sG + (size_t)cy * N;  // G_b[cx][cy][:] of product p, cell b at + (p*TB+b)*N2
       // One 4-step chunk at body offset jb (a compile-time constant after
@@ -204,7 +214,12 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
               for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int b = 0; b < TB; ++b) buf[p][b][slnew] = fl[p][b][-j];
+                for (int b = 0; b < TB; ++b) {
+                  // entry BS + base - j: with transposed lines BS + base = 4 B4 + 3
+                  // and fl points at slot B4 of sub-row 0
+                  if constexpr (Gm::TL) buf[p][b][slnew] = fl[p][b][((3 - j) & 3) * Gm::LS + ((3 - j) >> 2)];
+                  else buf[p][b][slnew] = fl[p][b][-j];
+                }
             }
           }
         }
@@ -219,7 +234,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-          for (int b = 0; b < TB; ++b) fl[p][b] = line[p][b] + BS + base;
+          for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + ((BS + base) >> 2) : line[p][b] + BS + base;
         const double2 *gz = gline + it * BS;
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
