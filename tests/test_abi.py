@@ -103,3 +103,26 @@ def test_initialize_weights_loads_reference_file_and_ghat3(spb, orc, tmp_path):
         r_idx = (i * 8 + j) * 8 + k; c_idx = (l * 8 + m) * 8 + n
         assert abs(w[l] * w[m] * w[n] * got - Wf[r_idx, c_idx]) <= 1e-13 * np.abs(Wf).max()
     L.dealloc_weights(8, rows)
+
+
+def test_run_and_net_entry_points_reject_bad_arguments(spb):
+    """Argument checks of the batched runs, transport and surrogate entry points
+    happen before any device call, so they are errors (not crashes) without a GPU."""
+    import ctypes
+    L = spb.lib()
+    h = ctypes.c_void_p()
+    dp = ctypes.POINTER(ctypes.c_double)
+    x = (ctypes.c_double * 8)()
+    assert L.spbte_run0d_create(ctypes.byref(h), None, 4, 2, 0.01, 1.0, 0, 1) == -1        # no operator
+    assert L.spbte_run1d_create(ctypes.byref(h), None, 4, 1, None, None, 0.01, 1.0, 1, 1.0, 2.0, 0) == -1  # no mesh
+    assert L.spbte_run1d_create(ctypes.byref(h), None, 4, 1, ctypes.cast(x, dp), ctypes.cast(x, dp), 0.01, 1.0, 7,
+                                1.0, 2.0, 0) == -1                                          # bad boundary
+    assert L.spbte_run1d_create(ctypes.byref(h), None, 4, 2, ctypes.cast(x, dp), ctypes.cast(x, dp), 0.01, 1.0, 2,
+                                1.0, 2.0, 0) == -7                                          # periodic at order 2
+    assert L.spbte_run_step(None, 1) == -1
+    assert L.spbte_run_set_surrogate(None, None, 8, 1e-3, 0.0) == -1
+    assert L.spbte_advect(None, None, None, 4, 1, None, None, 0.01, 1, 1.0, 2.0) == -1
+    assert L.spbte_net_create(ctypes.byref(h), None, 12, 64, 128, 4, 0, 1) == -1
+    assert L.spbte_net_collide(None, None, None, 4, 0, 8, 0, 1e-3, 0.0, None, None) == -1
+    assert L.spbte_net_train(None, 10, 8, 1e-3, 1, None) == -1
+    assert "bad" in L.spbte_last_error().decode() or "null" in L.spbte_last_error().decode()
