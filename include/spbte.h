@@ -107,11 +107,13 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
 
 /* Tiles of the weighted-sum kernels.  kind 0 is the tiled kernel (tr output rows
  * per lane, `lines` z-lines of output modes per workgroup, tb cells per
- * workgroup); kind 1 is the streaming kernel for large batches (tr rows per
- * lane, `lines` = waves per workgroup, tb cells per workgroup).  set_tile pins a
- * tile for (form, maxpreserve); tr = 0 restores automatic choice.  autotune
- * times every instantiated tile of both kinds on the given batch shape and pins
- * the fastest for that batch size; get_tile reports the tile a batch would use. */
+ * workgroup).  kind k >= 1 is the streaming kernel (tr rows per lane, `lines` =
+ * waves per workgroup, tb cells per workgroup) with the summed x-planes split
+ * into k slices whose partial sums are added afterwards (k > 1 gives small
+ * batches more workgroups; k must divide N).  set_tile pins a tile for (form,
+ * maxpreserve); tr = 0 restores automatic choice.  autotune times every
+ * instantiated tile on the given batch shape and pins the fastest for that
+ * batch size; get_tile reports the tile a batch would use. */
 int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int kind, int tr, int lines, int tb);
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb);
 int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb);

@@ -306,7 +306,8 @@ class Operator:
         return dQ.download().view(np.complex128).reshape(shape)
 
     def set_tile(self, form, maxpreserve, tr, lines, tb, kind=0):
-        """Pin a tile (kind 0: tiled kernel; kind 1: streaming kernel); tr=0 unpins."""
+        """Pin a tile (kind 0: tiled kernel; kind k >= 1: streaming kernel with the summed
+        x-planes in k slices); tr=0 unpins."""
         _check(lib().spbte_set_tile(self.h, FORMS[form], int(maxpreserve), int(kind), tr, lines, tb))
 
     def get_tile(self, ncell, form, maxpreserve=False):

@@ -80,7 +80,7 @@ template <int N, int TR, int WAVES, int TB, int NP, int FORM>
 __global__ void __launch_bounds__(64 * WAVES)
 k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
               const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
-              double2 *__restrict__ Qh, int ncell) {
+              double2 *__restrict__ Qh, int ncell, int csplit, double2 *__restrict__ Qp) {
   using Gm = SGeom<N, TR, WAVES>;
   constexpr int N2 = Gm::N2, P = Gm::P, RWG = Gm::RWG, NC4 = Gm::NC4;
   constexpr size_t n3 = (size_t)N * N * N;
@@ -92,15 +92,19 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
   __shared__ __attribute__((aligned(16))) double2 sG[NP * TB * N2];          // [p][b][cy][cz]
 
-  // ---- workgroup -> (r-tile, cell group), XCD-aware: workgroups are dispatched
-  // round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th contiguous
-  // slice of the r-tile-major list of (r-tile, cell group) pairs, and the
-  // workgroups it runs back to back share an r-tile and its W stream in its L2.
+  // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
+  // dispatched round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th
+  // contiguous slice of the r-tile-major list of (r-tile, cx slice, cell group)
+  // triples, and the workgroups it runs back to back share a piece of W in its
+  // L2.  With csplit > 1 the summed x-planes cx are split into csplit slices
+  // (small batches: more workgroups), each writing partial sums to Qp.
   const int ncg = (ncell + TB - 1) / TB;
-  const int total = NRT * ncg, per = (total + 7) >> 3;
+  const int total = NRT * csplit * ncg, per = (total + 7) >> 3;
   const int vid = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if ((blockIdx.x >> 3) >= per || vid >= total) return;  // the few padding workgroups of the last slice
-  const int rt = vid / ncg, cg = vid - rt * ncg;
+  const int rt = vid / (csplit * ncg), rem = vid - rt * (csplit * ncg);
+  const int cs = rem / ncg, cg = rem - cs * ncg;
+  const int cx0 = cs * (N / csplit), cx1 = cx0 + N / csplit;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t wg_row0 = (size_t)rt * RWG;
   const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
@@ -136,9 +140,9 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   // Two static register sets alternate over the 4-column chunks of W (a body
   // holds an even number of chunks), each loaded four cz steps before use.
   uint4 wA[TR][UPW], wB[TR][UPW];
-  wload(0, wA);
+  wload(cx0 * N * (N / 4), wA);
 
-  for (int cx = 0; cx < N; ++cx) {
+  for (int cx = cx0; cx < cx1; ++cx) {
     __syncthreads();
     // ---- stage the shifted F x-planes of every cell, product and lane plane,
     // each z-line extended to LW values (entry m = F[(m - BS) mod N])
@@ -259,24 +263,45 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
         s.x *= sc;
         s.y *= sc;
       }
-      Qh[(size_t)cell * n3 + row0 + t] = s;
+      if (csplit > 1) Qp[((size_t)cs * ncell + cell) * n3 + row0 + t] = s;
+      else Qh[(size_t)cell * n3 + row0 + t] = s;
     }
   }
 }
 
+// Qh[b][r] = sum over the csplit slices of Qp[s][b][r].
+__global__ void __launch_bounds__(256) k_slice_sum(size_t n, int csplit, const double2 *__restrict__ Qp,
+                                                   double2 *__restrict__ Qh) {
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+    double2 v = Qp[e];
+    for (int s = 1; s < csplit; ++s) {
+      const double2 w = Qp[(size_t)s * n + e];
+      v.x += w.x;
+      v.y += w.y;
+    }
+    Qh[e] = v;
+  }
+}
+
 template <int N, int TR, int WAVES, int TB, int NP, int FORM>
-int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, hipStream_t s) {
+int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, int csplit, void *Qp, hipStream_t s) {
   using Gm = SGeom<N, TR, WAVES>;
   const int ncg = (a->ncell + TB - 1) / TB;
   const int nrt = (int)((size_t)N * N * N / Gm::RWG);
-  const int per = (nrt * ncg + 7) / 8;  // workgroups per XCD slice
+  const int per = (nrt * csplit * ncg + 7) / 8;  // workgroups per XCD slice
   hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM>), dim3(8 * per), dim3(64 * WAVES), 0, s,
                      (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
-                     (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell);
+                     (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
+                     (double2 *)Qp);
+  if (csplit > 1) {
+    const size_t n = (size_t)a->ncell * N * N * N;
+    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384)), dim3(256), 0,
+                       s, n, csplit, (const double2 *)Qp, (double2 *)a->Qh);
+  }
   return (int)hipGetLastError();
 }
 
-typedef int (*launch_fn)(const spbte_wsum_args *, const void *, const float *, hipStream_t);
+typedef int (*launch_fn)(const spbte_wsum_args *, const void *, const float *, int, void *, hipStream_t);
 struct SEntry {
   int N, tr, waves, tb, np, form;
   launch_fn fn;
@@ -289,6 +314,8 @@ const SEntry kSTable[] = {
     // N = 32: TR=8 (4 lanes per z-line, 2 waves per x-plane) and TR=4 (8 lanes per z-line)
     S3(32, 8, 2, 1, 1), S3(32, 8, 2, 2, 1), S3(32, 4, 4, 2, 1), S3(32, 4, 2, 2, 1), S3(32, 4, 4, 4, 1),
     S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S3(32, 4, 4, 2, 2), S3(32, 4, 4, 1, 2),
+    // small batches: more, smaller r-tiles (half and quarter x-planes)
+    S3(32, 4, 2, 1, 1), S3(32, 4, 2, 1, 2), S3(32, 4, 1, 1, 1), S3(32, 4, 1, 2, 1),
     // N = 24 (rows per workgroup must divide 24^3): TR=3 x 3 waves = one x-plane,
     // TR=6 x 3 waves = 2 planes, TR=4 x 2 waves and TR=8 x 1 wave straddle planes
     S3(24, 3, 3, 1, 1), S3(24, 3, 3, 2, 1), S3(24, 3, 3, 1, 2), S3(24, 6, 3, 1, 1), S3(24, 6, 3, 1, 2),
@@ -349,11 +376,13 @@ extern "C" int spbte_stream_tile_supported(int N, int form, int nprod, int tr, i
 }
 
 // Launch with the permuted table Wp built for the same tr (spbte_launch_permute_weights).
+// csplit: slices of the summed x-planes (divides N); csplit > 1 needs Qp with
+// room for csplit * ncell * N^3 double2.
 extern "C" int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
-                                        const float *rowscale, hipStream_t s) {
+                                        const float *rowscale, int csplit, void *Qp, hipStream_t s) {
   const SEntry *e = sfind(N, form, a->nprod, tr, waves, tb);
-  if (!e) return (int)hipErrorInvalidValue;
-  return e->fn(a, Wp, rowscale, s);
+  if (!e || csplit < 1 || N % csplit || (csplit > 1 && !Qp)) return (int)hipErrorInvalidValue;
+  return e->fn(a, Wp, rowscale, csplit, Qp, s);
 }
 
 // rowscale (FP16x2 only) must already hold the per-row scales.

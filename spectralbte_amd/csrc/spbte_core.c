@@ -82,6 +82,9 @@ struct spbte_op {
   spbte_tile pin[NFORMS][2];
   /* autotuned tiles per (form, nprod-1, ncell bucket) */
   spbte_tile tuned[NFORMS][2][16];
+  /* partial sums of the split streaming kernel */
+  void *d_Qp;
+  size_t qp_bytes;
   /* bumped whenever a device buffer, the table, the species or a tile changes:
    * launch sequences captured in a graph are valid for one generation */
   unsigned gen;
@@ -311,6 +314,7 @@ static void free_scratch(spbte_op *op) {
 void spbte_op_destroy(spbte_op *op) {
   if (!op) return;
   if (op->s) hipStreamSynchronize(op->s);
+  if (op->d_Qp) hipFree(op->d_Qp);
   free_weights(op);
   free_scratch(op);
   if (op->g.d_mat_fwd) hipFree(op->g.d_mat_fwd);
@@ -543,12 +547,17 @@ static int default_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   return 0;
 }
 
-/* Streaming kernel for batches large enough to fill the chip with its
- * workgroups (one r-tile is a whole x-plane of output modes). */
+/* Workgroups of a streaming tile without the x-plane split. */
+static long stream_wgs(int N, int ncell, const spbte_tile *t) {
+  return (long)N * N * N / (64L * t->lines * t->tr) * ((ncell + t->tb - 1) / t->tb);
+}
+
+/* Streaming kernel defaults (measured on 256-512 cells, MI355X): FP64 weights
+ * favour 8 rows per lane for the two-product sum, FP32 weights 4 rows and 4
+ * waves.  Batches too small to give 2048 workgroups split the summed x-planes
+ * into up to 8 slices. */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   if (form == SPBTE_FORM_FP32A) return 0;
-  /* measured fastest on 256-512 cells (MI355X): FP64 weights favour 8 rows per
-   * lane for the two-product sum, FP32 weights 4 rows and 4 waves */
   static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}}, s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
@@ -563,9 +572,13 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   else return 0;
   for (int i = 0; i < nc; i++) {
     if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
-    const long rows_wg = 64L * c[i].lines * c[i].tr, nrt = (long)N * N * N / rows_wg;
-    const long wgs = nrt * ((ncell + c[i].tb - 1) / c[i].tb);
-    if (wgs >= 2048) { *t = c[i]; return 1; }
+    if (c[i].tb > ncell && i + 1 < nc) continue;
+    *t = c[i];
+    const long wgs = stream_wgs(N, ncell, t);
+    int k = 1;
+    while (k < 8 && wgs * k < 2048 && N % (2 * k) == 0) k *= 2;
+    t->kind = k;
+    return 1;
   }
   return 0;
 }
@@ -579,8 +592,23 @@ static int pick_tile(spbte_op *op, int ncell, int form, int nprod, spbte_tile *t
 }
 
 static int tile_supported(int N, int form, int nprod, const spbte_tile *t) {
-  if (t->kind == 1) return spbte_stream_tile_supported(N, form, nprod, t->tr, t->lines, t->tb);
+  if (t->kind >= 1)
+    return t->kind <= N && N % t->kind == 0 && spbte_stream_tile_supported(N, form, nprod, t->tr, t->lines, t->tb);
   return spbte_wsum_tile_supported(N, form, nprod, t);
+}
+
+
+/* Partial sums of the split streaming kernel: csplit * ncell * N^3 double2. */
+static int ensure_partials(spbte_op *op, size_t bytes) {
+  if (bytes <= op->qp_bytes) return SPBTE_OK;
+  HIPCHK(hipStreamSynchronize(op->s));
+  if (op->d_Qp) hipFree(op->d_Qp);
+  op->d_Qp = NULL;
+  op->qp_bytes = 0;
+  HIPCHK(hipMalloc(&op->d_Qp, bytes));
+  op->qp_bytes = bytes;
+  op->gen++;
+  return SPBTE_OK;
 }
 
 int spbte_set_tile(spbte_op *op, int form, int maxpreserve, int kind, int tr, int lines, int tb) {
@@ -606,11 +634,15 @@ int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
 /* ---------------------------------------------------------- the operator */
 static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a) {
   const float *rs;
-  if (t && t->kind == 1) {
+  if (t && t->kind >= 1) {
     const void *Wp = NULL;
     int rc = weights_permuted(op, form, t->tr, &Wp, &rs);
     if (rc) return rc;
-    LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, op->s));
+    if (t->kind > 1) {
+      rc = ensure_partials(op, (size_t)t->kind * a->ncell * op->n3 * 16);
+      if (rc) return rc;
+    }
+    LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, t->kind, op->d_Qp, op->s));
     return SPBTE_OK;
   }
   const void *W;
@@ -795,7 +827,10 @@ static const spbte_tile k_cand[] = {
     {4, 1, 1, 0}, {4, 2, 1, 0}, {4, 2, 2, 0}, {4, 4, 2, 0}, {4, 2, 4, 0}, {4, 4, 4, 0}, {8, 4, 2, 0}, {8, 8, 2, 0},
     {8, 4, 4, 0}, {6, 8, 1, 0}, {6, 8, 2, 0}, {6, 8, 4, 0}, {4, 8, 2, 0}, {4, 8, 4, 0}, {4, 4, 1, 0}, {2, 8, 2, 0},
     {8, 2, 1, 1}, {8, 2, 2, 1}, {4, 4, 2, 1}, {4, 2, 2, 1}, {4, 4, 4, 1}, {4, 4, 1, 1}, {6, 3, 1, 1}, {3, 3, 1, 1},
-    {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1}};
+    {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1},
+    /* streaming with the summed x-planes split in 2-8 slices, for small batches */
+    {4, 4, 1, 2}, {4, 4, 1, 4}, {4, 4, 1, 8}, {4, 4, 2, 2}, {4, 4, 2, 4}, {4, 4, 2, 8}, {4, 2, 1, 4}, {4, 2, 1, 8},
+    {4, 2, 2, 4}, {8, 2, 1, 4}, {8, 2, 1, 8}, {4, 2, 1, 2}, {4, 9, 1, 2}, {4, 9, 1, 4}};
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");
@@ -819,6 +854,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     const spbte_tile *t = &k_cand[i];
     if (!tile_supported(op->N, form, np, t)) continue;
     if (t->tb > 2 * ncell) continue;
+    if (t->kind > 1 && stream_wgs(op->N, ncell, t) >= 2048) continue; /* the chip is full without a split */
     rc = launch_tile(op, form, t, &args); /* warm-up; builds the table layout it needs */
     if (rc) break;
     HIPCHK(hipEventRecord(a, op->s));

@@ -52,9 +52,9 @@ typedef struct {
  * modes per workgroup); kind 1: k_wsum_stream.hip (lines = waves per workgroup). */
 typedef struct {
   int tr;    /* output rows (consecutive z modes) per lane */
-  int lines; /* z-lines (kind 0) or waves (kind 1) per workgroup */
+  int lines; /* z-lines (kind 0) or waves (kind >= 1) per workgroup */
   int tb;    /* cells per workgroup */
-  int kind;
+  int kind;  /* 0: tiled kernel; k >= 1: streaming kernel with the summed x-planes in k slices */
 } spbte_tile;
 
 /* ----- launchers (k_*.hip); all return hipError_t of the launch ----- */
@@ -73,7 +73,7 @@ int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
 int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb);
 int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
-                             const float *rowscale, hipStream_t s);
+                             const float *rowscale, int csplit, void *Qp, hipStream_t s);
 int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,
                                  hipStream_t s);
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,

@@ -97,6 +97,33 @@ def test_qhat_tiles_agree(spb, orc, N):
         assert rel_l2(op.qhat(F, G, "fp64"), ref) < 1e-12, t
 
 
+def test_n32_tiles_agree(spb, orc):
+    """Every N=32 tile of both kernel kinds, one and two products, gives the FP64
+    sum of the tiled kernel's (4 rows, 4 lines, 2 cells) tile, which
+    test_n32_single_cell_against_oracle checks against the oracle."""
+    N, Lv = 32, 6.0
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    rng = np.random.default_rng(32)
+    f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(3)])
+    tiles = [(4, 4, 2, 0), (8, 2, 1, 1), (8, 2, 2, 1), (4, 4, 2, 1), (4, 2, 2, 1), (4, 4, 4, 1), (4, 4, 1, 1),
+             (4, 2, 1, 1), (4, 1, 1, 1), (4, 1, 2, 1), (4, 4, 1, 2), (4, 4, 2, 4), (4, 2, 1, 8), (8, 2, 1, 4),
+             (4, 4, 1, 32)]
+    for mp in (False, True):
+        ref = None
+        for t in tiles:
+            try:
+                op.set_tile("fp64", mp, *t)
+            except spb.SpbteError:
+                continue  # not instantiated for this product count
+            Q = op.compute_q(f, maxpreserve=mp)
+            if ref is None:
+                ref = Q
+            assert rel_l2(Q, ref) < 1e-13, (mp, t)
+    op.close()
+
+
 @pytest.mark.parametrize("N", [16, 24])
 def test_maxpreserve_tiles_agree(spb, orc, N):
     """Every instantiated two-product (fused maxPreserve) tile against the oracle's
@@ -111,7 +138,8 @@ def test_maxpreserve_tiles_agree(spb, orc, N):
     f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(ncell)])
     ref = np.stack([orc.compute_q_maxpreserve(N, v, eta, W, f[b], f[b]).ravel() for b in range(ncell)])
     tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 1, 1, 1), (4, 1, 2, 1)],
-             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 3, 1, 1), (3, 3, 1, 1), (4, 2, 1, 1), (8, 1, 1, 1), (4, 9, 1, 1)]}[N]
+             24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 3, 1, 1), (3, 3, 1, 1), (4, 2, 1, 1), (8, 1, 1, 1), (4, 9, 1, 1),
+                  (4, 2, 1, 4), (4, 9, 1, 2), (3, 3, 1, 8)]}[N]
     for t in tiles:
         for form, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
             op.set_tile(form, True, *t)
