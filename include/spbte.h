@@ -165,6 +165,32 @@ int spbte_run_step(spbte_run *run, int nsteps);
 /* Per cell rho, u_x, u_y, u_z, T (src/momentRoutines.c:58-183): 5 * ncell host doubles. */
 int spbte_run_moments(spbte_run *run, double *out);
 
+/* Moment output in the reference's format (src/output.c:64-213, :215-404):
+ * a header line, then per spbte_run_write_output call one "%le "-column line
+ * per cell (0D: time, density, velocity_x, temperature, pressure, positive /
+ * negative energy ratio, slice f[N/2 + N(N/2 + N l)]; 1D: time, position,
+ * density, velocity_x, temperature, pressure).  flags select the columns as
+ * the reference's output-flags file does (read_flags, src/output.c:417-443);
+ * spbte_read_output_flags parses such a file.  One open file per run. */
+enum {
+  SPBTE_OUT_DENSITY = 1,
+  SPBTE_OUT_VELOCITY = 2,
+  SPBTE_OUT_TEMPERATURE = 4,
+  SPBTE_OUT_PRESSURE = 8,
+  SPBTE_OUT_ENTROPY = 16, /* 0D: positive / negative energy ratio */
+  SPBTE_OUT_SLICE = 32    /* 0D: the slice f[N/2 + N(N/2 + N l)], l = 0..N-1 */
+};
+int spbte_read_output_flags(const char *path, int *flags);
+int spbte_run_open_output(spbte_run *run, const char *path, int flags, int append);
+int spbte_run_write_output(spbte_run *run, double time);
+int spbte_run_close_output(spbte_run *run);
+
+/* Restart dumps (src/restart.c:24-109): the interior cells' N^3 doubles back
+ * to back in pdf_path, and the step counter as one int in time_path (either
+ * may be skipped with a null path on load). */
+int spbte_run_store_restart(spbte_run *run, const char *pdf_path, const char *time_path, int step);
+int spbte_run_load_restart(spbte_run *run, const char *pdf_path, const char *time_path, int *step);
+
 /* One transport stage alone, synchronous: order 1 = upwindOne, order 2 =
  * advectTwo, from d_f to d_fc, both (nX + 2*order) * N^3 device doubles with
  * ghost cells; d_f's ghost cells are overwritten as in the reference. */

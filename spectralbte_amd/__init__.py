@@ -132,6 +132,12 @@ def lib():
             "spbte_net_collide": (I, [_vp, _vp, _vp, I, I, I, I, D, D, ip, _dp]),
             "spbte_net_set_loss_floor": (I, [_vp, D]),
             "spbte_run_set_surrogate": (I, [_vp, _vp, I, D, D]),
+            "spbte_read_output_flags": (I, [ctypes.c_char_p, ip]),
+            "spbte_run_open_output": (I, [_vp, ctypes.c_char_p, I, I]),
+            "spbte_run_write_output": (I, [_vp, D]),
+            "spbte_run_close_output": (I, [_vp]),
+            "spbte_run_store_restart": (I, [_vp, ctypes.c_char_p, ctypes.c_char_p, I]),
+            "spbte_run_load_restart": (I, [_vp, ctypes.c_char_p, ctypes.c_char_p, ip]),
             "spbte_run_surrogate_stats": (I, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                                               ctypes.POINTER(ctypes.c_long), _dp]),
         }
@@ -331,6 +337,15 @@ class Operator:
 
 # ------------------------------------------------------ batched time stepping
 BC_COPY, BC_DIFFUSE, BC_PERIODIC = 0, 1, 2
+OUT_DENSITY, OUT_VELOCITY, OUT_TEMPERATURE, OUT_PRESSURE, OUT_ENTROPY, OUT_SLICE = 1, 2, 4, 8, 16, 32
+OUT_ALL = 63
+
+
+def read_output_flags(path):
+    """The column flags of a reference output-flags file (src/output.c:417-443)."""
+    f = ctypes.c_int(0)
+    _check(lib().spbte_read_output_flags(os.fsencode(path), ctypes.byref(f)))
+    return f.value
 
 
 class Run:
@@ -398,6 +413,25 @@ class Run:
         _check(lib().spbte_run_surrogate_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                                ctypes.byref(d)))
         return a.value, b.value, c.value, d.value
+
+    def open_output(self, path, flags=OUT_ALL, append=False):
+        _check(lib().spbte_run_open_output(self.h, os.fsencode(path), int(flags), int(append)))
+
+    def write_output(self, time):
+        _check(lib().spbte_run_write_output(self.h, float(time)))
+
+    def close_output(self):
+        _check(lib().spbte_run_close_output(self.h))
+
+    def store_restart(self, pdf_path, time_path=None, step=0):
+        _check(lib().spbte_run_store_restart(self.h, os.fsencode(pdf_path),
+                                             os.fsencode(time_path) if time_path else None, int(step)))
+
+    def load_restart(self, pdf_path, time_path=None):
+        step = ctypes.c_int(0)
+        _check(lib().spbte_run_load_restart(self.h, os.fsencode(pdf_path),
+                                            os.fsencode(time_path) if time_path else None, ctypes.byref(step)))
+        return step.value
 
     def moments(self):
         """(ncell, 5): rho, u_x, u_y, u_z, T per cell."""

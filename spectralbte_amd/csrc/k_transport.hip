@@ -228,3 +228,45 @@ extern "C" int spbte_launch_lincomb(size_t n, double a, const double *x, double 
   hipLaunchKernelGGL(k_lincomb, dim3(blocks(n)), dim3(ET), 0, s, n, a, x, b, y, c, z, out);
   return (int)hipGetLastError();
 }
+
+// ---- output quantities of src/output.c per cell: out[c] = {E+, E-, slice[0..N)}
+// with E+- the positive and negative parts of sum dv^3 w w w f |v|^2
+// (src/momentRoutines.c:146-165) and slice[l] = f[N/2 + N (N/2 + N l)]
+// (src/output.c:203-206).  One workgroup per cell.
+__global__ void __launch_bounds__(ET) k_out_quantities(int N, const double *__restrict__ f, const double *__restrict__ v,
+                                                       const double *__restrict__ w, double hv,
+                                                       double *__restrict__ out) {
+  __shared__ double red[2][ET / 64];
+  const int c = blockIdx.x;
+  const size_t n3 = (size_t)N * N * N;
+  const double *fc = f + c * n3;
+  const double h3 = hv * hv * hv;
+  double pos = 0.0, neg = 0.0;
+  for (int e = threadIdx.x; e < (int)n3; e += ET) {
+    const int i = e / (N * N), j = (e / N) % N, k = e % N;
+    const double en = h3 * w[i] * w[j] * w[k] * fc[e] * (v[i] * v[i] + v[j] * v[j] + v[k] * v[k]);
+    if (en > 0) pos += en;
+    else neg -= en;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    pos += __shfl_down(pos, off, 64);
+    neg += __shfl_down(neg, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = pos; red[1][threadIdx.x >> 6] = neg; }
+  __syncthreads();
+  double *o = out + (size_t)c * (2 + N);
+  if (threadIdx.x == 0) {
+    double p = 0.0, q = 0.0;
+    for (int s = 0; s < ET / 64; ++s) { p += red[0][s]; q += red[1][s]; }
+    o[0] = p;
+    o[1] = q;
+  }
+  for (int l = threadIdx.x; l < N; l += ET) o[2 + l] = fc[N / 2 + (size_t)N * (N / 2 + (size_t)N * l)];
+}
+
+extern "C" int spbte_launch_out_quantities(const spbte_grid_dev *g, const double *f, int ncell, double *out,
+                                           hipStream_t s) {
+  hipLaunchKernelGGL(k_out_quantities, dim3(ncell), dim3(ET), 0, s, g->N, f, (const double *)g->d_v,
+                     (const double *)g->d_w, g->hv, out);
+  return (int)hipGetLastError();
+}

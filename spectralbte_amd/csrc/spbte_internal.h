@@ -94,6 +94,11 @@ int spbte_launch_upwind2(int N, int nX, const double *f, double *fc, const doubl
                          const double *v, const double *x, const double *dx, double dt, hipStream_t s);
 int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const double *y, double c, const double *z,
                          double *out, hipStream_t s);
+int spbte_launch_out_quantities(const spbte_grid_dev *g, const double *f, int ncell, double *out, hipStream_t s);
+/* run internals used by the output module (spbte_run.c) */
+struct spbte_run;
+int spbte_run_geometry(struct spbte_run *run, struct spbte_op **op, int *dim, int *ncell, const double **d_state,
+                       const double **d_x);
 
 /* learned surrogate kernels (k_net.hip) */
 int spbte_net_gemm(const float *A, long sa0, long sa1, const float *B, long sb0, long sb1, float *C, long sc0, long sc1,

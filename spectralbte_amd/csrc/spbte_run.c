@@ -372,3 +372,13 @@ int spbte_advect(spbte_op *op, double *d_f, double *d_fc, int nX, int order, con
   spbte_run_destroy(r);
   return rc;
 }
+
+int spbte_run_geometry(spbte_run *r, spbte_op **op, int *dim, int *ncell, const double **d_state, const double **d_x) {
+  if (!r) return spbte_fail(SPBTE_ERR_ARG, "null argument");
+  *op = r->op;
+  *dim = r->dim;
+  *ncell = r->ncell;
+  *d_state = interior(r, r->d_f);
+  *d_x = r->d_x ? r->d_x + r->nghost : NULL;
+  return SPBTE_OK;
+}

@@ -126,3 +126,16 @@ def test_run_and_net_entry_points_reject_bad_arguments(spb):
     assert L.spbte_net_collide(None, None, None, 4, 0, 8, 0, 1e-3, 0.0, None, None) == -1
     assert L.spbte_net_train(None, 10, 8, 1e-3, 1, None) == -1
     assert "bad" in L.spbte_last_error().decode() or "null" in L.spbte_last_error().decode()
+
+
+def test_output_flags_file(spb, tmp_path):
+    """read_flags of src/output.c:417-443 on the reference's own flag files (all
+    columns on), and a file that switches two columns off."""
+    for case in ("BKW8/BKW8.test.out", "heat_transport/heat_transport.test.out"):
+        assert spb.read_output_flags(os.path.join(GOLDEN, case)) == spb.OUT_ALL
+    p = tmp_path / "flags.out"
+    p.write_text("# flags\ndensity\n1\nvelocity\n0\ntemperature\n1\nslice\n0\nStop\n")
+    assert spb.read_output_flags(str(p)) == spb.OUT_ALL & ~spb.OUT_VELOCITY & ~spb.OUT_SLICE
+    p.write_text("density\n1\n")
+    with pytest.raises(spb.SpbteError):
+        spb.read_output_flags(str(p))

@@ -225,3 +225,70 @@ def test_run0d_reduced_form_and_conservation(spb, orc):
         dm = np.abs((st - f0) @ C.T)
         assert dm.max() < 1e-12 * (np.abs(f0) @ np.abs(C.T)).max()
     r64.close(); r32.close(); op.close()
+
+
+def _numeric_rows(path):
+    rows = []
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        rows.append([float(x) for x in line.split()])
+    return np.array(rows)
+
+
+def test_output_file_matches_bkw8_fixture(spb, orc, tmp_path):
+    """The moment file of src/output.c written by a batched 0D run has the
+    reference fixture's header and columns, within its print precision."""
+    N, Lv, dt = 8, 5.0, 0.01
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_weights(orc.weights_load(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), N))
+    run = spb.Run.ensemble(op, orc.bkw_initial(N, v).ravel()[None, :], order=2, dt=dt)
+    out = tmp_path / "moments_BKW8"
+    run.open_output(str(out), spb.read_output_flags(os.path.join(GOLDEN, "BKW8", "BKW8.test.out")))
+    run.write_output(0.0)
+    for t in range(100):
+        run.step(1)
+        run.write_output(dt * (t + 1))
+    run.close_output()
+    ref_path = os.path.join(GOLDEN, "BKW8", "moments_BKW8.test.in")
+    assert open(out).readline() == open(ref_path).readline()
+    got, ref = _numeric_rows(out), _numeric_rows(ref_path)
+    assert got.shape == ref.shape
+    m = np.abs(ref) > 1e-12
+    assert (np.abs(got - ref)[m] / np.abs(ref)[m]).max() < 5e-7
+    run.close()
+    op.close()
+
+
+def test_output_file_matches_heat_fixture_and_restart(spb, orc, tmp_path):
+    """The 1D moment file (time, position, density, velocity, temperature,
+    pressure per cell) of the heat-transport run against its fixture, and a
+    restart dump (src/restart.c) that reloads bit for bit."""
+    N, Lv, nX, dt, Kn = 8, 9.0, 250, 1e-4, 3.2
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_weights(orc.weights_load(os.path.join(GOLDEN, "heat_transport", "N8_isotropic_L_v9_lambda1.wts"), N))
+    x, dx = orc.uniform_mesh(nX, 1.0, 1)
+    f0 = np.tile(orc.maxwellian_1d(v, 1.0, 0.0, 1.5).ravel(), (nX, 1))
+    run = spb.Run.mesh(op, f0, x, dx, order=1, dt=dt, Kn=Kn, bc=spb.BC_DIFFUSE)
+    out = tmp_path / "moments_heat"
+    run.open_output(str(out), spb.read_output_flags(os.path.join(GOLDEN, "heat_transport", "heat_transport.test.out")))
+    run.write_output(0.0)
+    for t in range(10):
+        run.step(1)
+        run.write_output(dt * (t + 1))
+    run.close_output()
+    ref_path = os.path.join(GOLDEN, "heat_transport", "moments_heat_transport.test.in")
+    assert open(out).readline() == open(ref_path).readline()
+    got, ref = _numeric_rows(out), _numeric_rows(ref_path)
+    assert got.shape == ref.shape
+    for c in (0, 1, 2, 4, 5):       # time, position, density, temperature, pressure
+        assert (np.abs(got[:, c] - ref[:, c]) / np.maximum(np.abs(ref[:, c]), 1e-30)).max() < 5e-7, c
+    pdf, tim = tmp_path / "r.plt", tmp_path / "r_time.plt"
+    run.store_restart(str(pdf), str(tim), 10)
+    again = spb.Run.mesh(op, np.zeros_like(f0), x, dx, order=1, dt=dt, Kn=Kn, bc=spb.BC_DIFFUSE)
+    assert again.load_restart(str(pdf), str(tim)) == 10
+    assert np.array_equal(again.state(), run.state())
+    assert os.path.getsize(pdf) == nX * N ** 3 * 8
+    run.close(); again.close(); op.close()
