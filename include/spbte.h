@@ -86,6 +86,9 @@ int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows);
 int spbte_op_load_weights_file(spbte_op *op, const char *path);
 int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, double d_j, double m_i, double m_j);
 int spbte_op_get_weights(spbte_op *op, double *W);
+/* Writes the FP64 table as a Weights/ file (N^3 rows of N^3 doubles, the
+ * layout src/weights.c:95-104 stores). */
+int spbte_op_store_weights_file(spbte_op *op, const char *path);
 
 /* Q for ncell cells: d_Q[b] = Q(d_f[b], d_g[b]) (ComputeQ, or maxPreserve with
  * SPBTE_MAXPRESERVE), optionally projected (SPBTE_CONSERVE). */
@@ -140,12 +143,15 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
  * step, advectTwo.  Walls: SPBTE_BC_DIFFUSE = diffuse reflection at T0 (left)
  * and T1 (right) (src/boundaryConditions.c:39-85, ICChoice 3); SPBTE_BC_COPY =
  * ghosts copy their neighbours at order 1 and the no-flux wall states at
- * order 2; SPBTE_BC_PERIODIC = periodic ghosts (order 1, ICChoice 6).
+ * order 2; SPBTE_BC_PERIODIC = periodic ghosts (order 1, ICChoice 6);
+ * SPBTE_BC_DIFFUSE_LEFT = diffuse reflection at T0 on the left wall only, the
+ * right wall as SPBTE_BC_COPY (ICChoice 1, src/transportroutines.c:120-121,
+ * :363-364).
  *
  * spbte_run_use_graph(run, 1) captures one step as a HIP graph on first use
  * and replays it; the graph is captured again whenever the operator's buffers,
  * table, species or tiles change.  The run must not outlive its operator. */
-enum { SPBTE_BC_COPY = 0, SPBTE_BC_DIFFUSE = 1, SPBTE_BC_PERIODIC = 2 };
+enum { SPBTE_BC_COPY = 0, SPBTE_BC_DIFFUSE = 1, SPBTE_BC_PERIODIC = 2, SPBTE_BC_DIFFUSE_LEFT = 3 };
 
 typedef struct spbte_run spbte_run;
 
@@ -162,6 +168,11 @@ int spbte_run_state(spbte_run *run, double **d_f);
 int spbte_run_use_graph(spbte_run *run, int on);
 /* Queue nsteps steps (asynchronous; spbte_op_synchronize waits). */
 int spbte_run_step(spbte_run *run, int nsteps);
+/* Poiseuille body force of magnitude Ma along v_y in every transport stage of
+ * a 1D order-2 run (ICChoice 5, src/transportroutines.c:429-437, :458-466);
+ * Ma = 0 turns it off.  Order-1 runs are refused: the reference applies that
+ * term after its k loop, outside the cell (:218-226). */
+int spbte_run_set_forcing(spbte_run *run, double Ma);
 /* Per cell rho, u_x, u_y, u_z, T (src/momentRoutines.c:58-183): 5 * ncell host doubles. */
 int spbte_run_moments(spbte_run *run, double *out);
 
@@ -171,7 +182,9 @@ int spbte_run_moments(spbte_run *run, double *out);
  * negative energy ratio, slice f[N/2 + N(N/2 + N l)]; 1D: time, position,
  * density, velocity_x, temperature, pressure).  flags select the columns as
  * the reference's output-flags file does (read_flags, src/output.c:417-443);
- * spbte_read_output_flags parses such a file.  One open file per run. */
+ * spbte_read_output_flags parses such a file.  One open file per run; append
+ * opens an existing file for a restarted run (the header line is written in
+ * both cases, as the reference does). */
 enum {
   SPBTE_OUT_DENSITY = 1,
   SPBTE_OUT_VELOCITY = 2,

@@ -62,8 +62,8 @@ def lib():
             "orc_weights_load": (I, [ctypes.c_char_p, I, _dp]),
             "orc_diffuse_bc": (None, [I, _dp, D, _dp, _dp, D, I]),
             "orc_upwind_one": (None, [I, I, _dp, D, _dp, D, I, D, D, _dp, _dp]),
-            "orc_upwind_two": (None, [I, I, _dp, D, _dp, _dp, D, I, D, D, _dp, _dp]),
-            "orc_advect_two": (None, [I, I, _dp, D, _dp, _dp, D, I, D, D, _dp, _dp, _dp]),
+            "orc_upwind_two": (None, [I, I, _dp, D, _dp, _dp, D, I, D, D, D, _dp, _dp]),
+            "orc_advect_two": (None, [I, I, _dp, D, _dp, _dp, D, I, D, D, D, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -211,18 +211,18 @@ def upwind_one(N, nX, v, dx, dt, bc, f, T0=1.0, T1=2.0, mass=1.0):
     return f, fc
 
 
-def upwind_two(N, nX, v, x, dx, dt, bc, f, T0=1.0, T1=2.0, mass=1.0):
+def upwind_two(N, nX, v, x, dx, dt, bc, f, T0=1.0, T1=2.0, mass=1.0, Ma=0.0):
     f = _c(f).copy()
     fc = np.zeros_like(f)
-    lib().orc_upwind_two(N, nX, _p(_c(v)), mass, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1, _p(f), _p(fc))
+    lib().orc_upwind_two(N, nX, _p(_c(v)), mass, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1, Ma, _p(f), _p(fc))
     return f, fc
 
 
-def advect_two(N, nX, v, x, dx, dt, bc, f, T0=1.0, T1=2.0, mass=1.0):
+def advect_two(N, nX, v, x, dx, dt, bc, f, T0=1.0, T1=2.0, mass=1.0, Ma=0.0):
     f = _c(f).copy()
     fc = np.zeros_like(f)
     tmp = np.zeros_like(f)
-    lib().orc_advect_two(N, nX, _p(_c(v)), mass, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1, _p(f), _p(fc), _p(tmp))
+    lib().orc_advect_two(N, nX, _p(_c(v)), mass, _p(_c(x)), _p(_c(dx)), dt, bc, T0, T1, Ma, _p(f), _p(fc), _p(tmp))
     return f, fc
 
 
@@ -297,7 +297,7 @@ def uniform_mesh(nX, length, order):
 
 
 def run_1d(N, v, eta, W, f0, x, dx, dt, nsteps, order, Kn, bc, T0=1.0, T1=2.0, out_every=1, nthreads=None,
-           callback=None):
+           callback=None, Ma=0.0):
     """exec/boltz.c:254-345 for one species on one rank.  f0: (nX, N^3).
     Returns list of snapshots, each (nX, 4) = rho, u_x, T, p per cell."""
     nX = f0.shape[0]
@@ -321,7 +321,7 @@ def run_1d(N, v, eta, W, f0, x, dx, dt, nsteps, order, Kn, bc, T0=1.0, T1=2.0, o
         if order == 1:
             f, fc = upwind_one(N, nX, v, dx, dt, bc, f, T0, T1)
         else:
-            f, fc = advect_two(N, nX, v, x, dx, dt, bc, f, T0, T1)
+            f, fc = advect_two(N, nX, v, x, dx, dt, bc, f, T0, T1, Ma=Ma)
         for l in range(order, nX + order):
             Q = cons(compute_q(N, v, eta, W, fc[l], fc[l], nthreads=nthreads).reshape(-1))
             if order == 1:
@@ -332,7 +332,7 @@ def run_1d(N, v, eta, W, f0, x, dx, dt, nsteps, order, Kn, bc, T0=1.0, T1=2.0, o
                 fc[l] = 0.5 * fc[l] + 0.5 * f1 + 0.5 * dt * Q / Kn
         if order == 2:
             fc2 = fc.copy()
-            fc2, fnew = advect_two(N, nX, v, x, dx, dt, bc, fc2, T0, T1)
+            fc2, fnew = advect_two(N, nX, v, x, dx, dt, bc, fc2, T0, T1, Ma=Ma)
             f[order:order + nX] = fnew[order:order + nX]
         if callback is not None:
             callback(t, f)

@@ -584,8 +584,9 @@ void orc_diffuse_bc(int N, const double *v, double mass, const double *in, doubl
 /* src/transportroutines.c:94-238 (upwindOne) for a single rank.  f and fc
  * hold nX+2 cells of N^3 (ghosts 0 and nX+1).  bc: 0 = copy neighbour
  * (f[0] = f[1], f[nX+1] = f[nX]), 1 = diffuse walls at T0 (left) and T1
- * (right) (ICChoice 3/5), 2 = periodic (ICChoice 6). Writes fc[1..nX].
- * f's ghost cells are overwritten. */
+ * (right) (ICChoice 3/5), 2 = periodic (ICChoice 6), 3 = diffuse left wall at
+ * T0 and copied right ghost (ICChoice 1, :120-121, :136-137). Writes
+ * fc[1..nX].  f's ghost cells are overwritten. */
 void orc_upwind_one(int N, int nX, const double *v, double mass, const double *dx, double dt, int bc, double T0, double T1,
                     double *f, double *fc) {
   long n3 = (long)N * N * N;
@@ -595,6 +596,9 @@ void orc_upwind_one(int N, int nX, const double *v, double mass, const double *d
   } else if (bc == 2) {
     memcpy(f, f + nX * n3, sizeof(double) * n3);
     memcpy(f + (nX + 1) * n3, f + 1 * n3, sizeof(double) * n3);
+  } else if (bc == 3) {
+    orc_diffuse_bc(N, v, mass, f + 1 * n3, f + 0, T0, 0);
+    memcpy(f + (nX + 1) * n3, f + nX * n3, sizeof(double) * n3);
   } else {
     memcpy(f, f + 1 * n3, sizeof(double) * n3);
     memcpy(f + (nX + 1) * n3, f + nX * n3, sizeof(double) * n3);
@@ -615,11 +619,21 @@ void orc_upwind_one(int N, int nX, const double *v, double mass, const double *d
  * nX+2,nX+3); x and dx hold nX+4 entries (the right ghosts follow the
  * last-rank formula of src/mesh_setup.c:165-175; the reference leaves them
  * unset on a single rank).  Ghost 1 and nX+2 are linearly extrapolated
- * (:271-274, :293-296).  bc: 1 = diffuse walls (ICChoice 3/5), otherwise the
- * no-net-flux wall states of :366-377 and :392-403.  Writes fc[2..nX+1]. */
+ * (:271-274, :293-296).  bc: 1 = diffuse walls (ICChoice 3/5), 3 = diffuse
+ * left wall only (ICChoice 1, :363-364), otherwise the no-net-flux wall
+ * states of :366-377 and :392-403.  Ma != 0 adds the Poiseuille body force
+ * of :429-437 and :458-466 (coefficient Ma*0.5*dt/(2*h_v), h_v = 2 L_v/(N-1),
+ * with its one-sided ends as written there).  Writes fc[2..nX+1]. */
+static void force_vy(int N, double fy, const double *f0, long t, int j, double *out) {
+  if (j == 0) *out = *out - fy * f0[t + N];
+  else if (j == N - 1) *out = *out - fy * f0[t - N];
+  else *out = *out - fy * (f0[t + N] - f0[t - N]);
+}
+
 void orc_upwind_two(int N, int nX, const double *v, double mass, const double *x, const double *dx, double dt, int bc,
-                    double T0, double T1, double *f, double *fc) {
+                    double T0, double T1, double Ma, double *f, double *fc) {
   long n3 = (long)N * N * N;
+  const double h_v = 2 * (-v[0]) / (N - 1), fy = Ma * 0.5 * dt / (2 * h_v);
   double *fl = malloc(sizeof(double) * n3), *fr = malloc(sizeof(double) * n3);
   for (long t = 0; t < n3; t++) {
     f[1 * n3 + t] = 2 * f[2 * n3 + t] - f[3 * n3 + t];
@@ -635,7 +649,7 @@ void orc_upwind_two(int N, int nX, const double *v, double mass, const double *x
                                (f[3 * n3 + t] - f[1 * n3 + t]) / (x[3] - x[1]));
             fl[t] = f[2 * n3 + t] - 0.5 * dx[2] * s;
           }
-      if (bc == 1) orc_diffuse_bc(N, v, mass, fl, fl, T0, 0);
+      if (bc == 1 || bc == 3) orc_diffuse_bc(N, v, mass, fl, fl, T0, 0);
       else
         for (int j = 0; j < N; j++)
           for (int k = 0; k < N; k++)
@@ -682,6 +696,7 @@ void orc_upwind_two(int N, int nX, const double *v, double mass, const double *x
                                 (f0[t] - fm2[t]) / (x[l] - x[l - 2]));
             fc[l * n3 + t] = f0[t] - cfl * (f0[t] + 0.5 * dx[l] * s1 - (fm1[t] + 0.5 * dx[l - 1] * s0));
           }
+          if (Ma != 0.0) force_vy(N, fy, f0, t, j, &fc[l * n3 + t]);
         }
         for (int i = 0; i < N / 2; i++) {
           long t = IDX3(N, i, j, k);
@@ -695,6 +710,7 @@ void orc_upwind_two(int N, int nX, const double *v, double mass, const double *x
                                 (fp2[t] - f0[t]) / (x[l + 2] - x[l]));
             fc[l * n3 + t] = f0[t] - cfl * (fp1[t] - 0.5 * dx[l + 1] * s2 - (f0[t] - 0.5 * dx[l] * s1));
           }
+          if (Ma != 0.0) force_vy(N, fy, f0, t, j, &fc[l * n3 + t]);
         }
       }
   }
@@ -704,10 +720,10 @@ void orc_upwind_two(int N, int nX, const double *v, double mass, const double *x
 /* src/transportroutines.c:477-492 (advectTwo): two upwindTwo stages and the
  * average fc = (f + stage2)/2 over interior cells 2..nX+1.  tmp holds nX+4 cells. */
 void orc_advect_two(int N, int nX, const double *v, double mass, const double *x, const double *dx, double dt, int bc,
-                    double T0, double T1, double *f, double *fc, double *tmp) {
+                    double T0, double T1, double Ma, double *f, double *fc, double *tmp) {
   long n3 = (long)N * N * N;
-  orc_upwind_two(N, nX, v, mass, x, dx, dt, bc, T0, T1, f, tmp);
-  orc_upwind_two(N, nX, v, mass, x, dx, dt, bc, T0, T1, tmp, fc);
+  orc_upwind_two(N, nX, v, mass, x, dx, dt, bc, T0, T1, Ma, f, tmp);
+  orc_upwind_two(N, nX, v, mass, x, dx, dt, bc, T0, T1, Ma, tmp, fc);
   for (int l = 2; l < nX + 2; l++)
     for (long t = 0; t < n3; t++) fc[l * n3 + t] = 0.5 * (f[l * n3 + t] + fc[l * n3 + t]);
 }

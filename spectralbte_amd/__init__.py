@@ -140,6 +140,16 @@ def lib():
             "spbte_run_load_restart": (I, [_vp, ctypes.c_char_p, ctypes.c_char_p, ip]),
             "spbte_run_surrogate_stats": (I, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                                               ctypes.POINTER(ctypes.c_long), _dp]),
+            "spbte_run_set_forcing": (I, [_vp, D]),
+            "spbte_op_store_weights_file": (I, [_vp, ctypes.c_char_p]),
+            # front end (spbte_boltz.h)
+            "spbte_read_input": (I, [ctypes.c_char_p, _vp]),
+            "spbte_species_defaults": (None, [_vp, I]),
+            "spbte_read_species": (I, [ctypes.c_char_p, _vp]),
+            "spbte_read_mesh": (I, [ctypes.c_char_p, I, ip, _dp, _dp, _dp]),
+            "spbte_velocity_grid": (None, [I, D, I, _dp, _dp]),
+            "spbte_initial_state": (I, [_vp, _vp, _dp, I, _dp]),
+            "spbte_boltz_main": (I, [I, ctypes.POINTER(ctypes.c_char_p)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -248,6 +258,9 @@ class Operator:
     def generate_weights_iso(self, lam, d_i=2.0, d_j=2.0, m_i=1.0, m_j=1.0):
         _check(lib().spbte_op_generate_weights_iso(self.h, lam, d_i, d_j, m_i, m_j))
 
+    def store_weights_file(self, path):
+        _check(lib().spbte_op_store_weights_file(self.h, os.fsencode(path)))
+
     def get_weights(self):
         W = np.empty((self.n3, self.n3))
         _check(lib().spbte_op_get_weights(self.h, _p(W)))
@@ -336,7 +349,7 @@ class Operator:
 
 
 # ------------------------------------------------------ batched time stepping
-BC_COPY, BC_DIFFUSE, BC_PERIODIC = 0, 1, 2
+BC_COPY, BC_DIFFUSE, BC_PERIODIC, BC_DIFFUSE_LEFT = 0, 1, 2, 3
 OUT_DENSITY, OUT_VELOCITY, OUT_TEMPERATURE, OUT_PRESSURE, OUT_ENTROPY, OUT_SLICE = 1, 2, 4, 8, 16, 32
 OUT_ALL = 63
 
@@ -402,6 +415,10 @@ class Run:
         _check(lib().spbte_run_step(self.h, int(nsteps)))
         if sync:
             self.op.sync()
+
+    def set_forcing(self, Ma):
+        """Poiseuille body force along v_y (1D order-2 runs, Init_field 5)."""
+        _check(lib().spbte_run_set_forcing(self.h, float(Ma)))
 
     def set_surrogate(self, net, check_every=8, tol=1e-3, floor_rel=0.0):
         _check(lib().spbte_run_set_surrogate(self.h, net.h if net is not None else None, int(check_every), float(tol),
@@ -697,3 +714,67 @@ def solveWithCCt(b):
     b = _c(b).copy()
     lib().solveWithCCt(len(b), _p(b))
     return b
+
+
+# ------------------------------------------------ boltz_ front end (spbte_boltz.h)
+class Input(ctypes.Structure):
+    """Binary image of spbte_input: the keywords of a reference input file."""
+    _fields_ = [("N", ctypes.c_int), ("L_v", ctypes.c_double), ("Kn", ctypes.c_double), ("lambda_", ctypes.c_double),
+                ("dt", ctypes.c_double), ("nT", ctypes.c_int), ("order", ctypes.c_int), ("data_freq", ctypes.c_int),
+                ("restart", ctypes.c_int), ("restart_time", ctypes.c_double), ("init_flag", ctypes.c_int),
+                ("bc_flag", ctypes.c_int), ("inhom", ctypes.c_int), ("weight_flag", ctypes.c_int),
+                ("aniso", ctypes.c_int), ("mesh_file", ctypes.c_char * 80), ("num_species", ctypes.c_int),
+                ("species_names", (ctypes.c_char * 80) * 8)]
+
+
+def read_input(path):
+    """The keywords of a reference input file (src/input.c:10-183)."""
+    inp = Input()
+    _check(lib().spbte_read_input(os.fsencode(path), ctypes.byref(inp)))
+    return inp
+
+
+def read_species(path):
+    """A species file (src/species.c:64-159) over the defaults of :28-43."""
+    sp = Species()
+    lib().spbte_species_defaults(ctypes.byref(sp), 0)
+    _check(lib().spbte_read_species(os.fsencode(path), ctypes.byref(sp)))
+    return sp
+
+
+def read_mesh(path, order):
+    """(x, dx, dx_min) of a mesh file (src/mesh_setup.c:9-200), nX + 2*order entries."""
+    n = ctypes.c_int(0)
+    _check(lib().spbte_read_mesh(os.fsencode(path), int(order), ctypes.byref(n), None, None, None))
+    x, dx, dmin = np.empty(n.value + 2 * order), np.empty(n.value + 2 * order), ctypes.c_double()
+    _check(lib().spbte_read_mesh(os.fsencode(path), int(order), ctypes.byref(n), _p(x), _p(dx), ctypes.byref(dmin)))
+    return x, dx, dmin.value
+
+
+def velocity_grid(N, L_v, inhom):
+    """(v, eta) of initialize_hom (inhom=0) or initialize_inhom (inhom=1)."""
+    v, eta = np.empty(N), np.empty(N)
+    lib().spbte_velocity_grid(int(N), float(L_v), int(inhom), _p(v), _p(eta))
+    return v, eta
+
+
+def initial_state(inp, species, v, nX=1):
+    """The Init_field state of an Input: (N^3,) for 0D, (nX, N^3) for 1D."""
+    n3 = inp.N ** 3
+    f = np.empty((nX if inp.inhom else 1) * n3)
+    _check(lib().spbte_initial_state(ctypes.byref(inp), ctypes.byref(species), _p(_c(v)), int(nX), _p(f)))
+    return f.reshape(nX, n3) if inp.inhom else f
+
+
+def boltz(input_name, flags_name, workdir="."):
+    """Runs spbte_boltz_main (exec/boltz.c) in workdir, which holds input/,
+    Weights/ and Data/ as for the reference's boltz_.  Raises on failure."""
+    prev = os.getcwd()
+    argv = (ctypes.c_char_p * 3)(b"boltz_amd", os.fsencode(input_name), os.fsencode(flags_name))
+    os.chdir(workdir)
+    try:
+        rc = lib().spbte_boltz_main(3, argv)
+    finally:
+        os.chdir(prev)
+    if rc != 0:
+        raise SpbteError(lib().spbte_last_error().decode())

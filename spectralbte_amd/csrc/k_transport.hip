@@ -108,6 +108,8 @@ __global__ void __launch_bounds__(WT) k_diffuse(int N, WallJobs J, const double 
 // ---- order 2, wall states (upwindTwo :349-404).  fl: left-wall state from
 // cell 2 (outgoing half always; incoming half too unless diffuse walls, where
 // k_diffuse sets it), fr: right-wall state from cell nX+1, symmetrically.
+// diffuse: bit 0 = the left wall reflects diffusely, bit 1 = the right wall;
+// a diffuse wall's incoming half is left for k_diffuse.
 __global__ void __launch_bounds__(ET) k_walls2(int N, int nX, int diffuse, const double *__restrict__ f,
                                                double *__restrict__ fl, double *__restrict__ fr,
                                                const double *__restrict__ x, const double *__restrict__ dx) {
@@ -119,12 +121,12 @@ __global__ void __launch_bounds__(ET) k_walls2(int N, int nX, int diffuse, const
   const int i = (int)(t / ((size_t)N * N));
   if (side == 0) {
     const bool out_half = i < N / 2;
-    if (!out_half && diffuse) return;
+    if (!out_half && (diffuse & 1)) return;
     const double s = slope(f, n3, t, 2, x);
     fl[t] = out_half ? f[2 * n3 + t] - 0.5 * dx[2] * s : f[2 * n3 + t] + 0.5 * dx[2] * s;
   } else {
     const bool out_half = i >= N / 2;
-    if (!out_half && diffuse) return;
+    if (!out_half && (diffuse & 2)) return;
     const double s = slope(f, n3, t, nX + 1, x);
     const double c = f[(size_t)(nX + 1) * n3 + t];
     fr[t] = out_half ? c + 0.5 * dx[nX + 1] * s : c - 0.5 * dx[nX + 1] * s;
@@ -132,10 +134,13 @@ __global__ void __launch_bounds__(ET) k_walls2(int N, int nX, int diffuse, const
 }
 
 // ---- order 2, MUSCL half-step (upwindTwo :406-468) for l = 2..nX+1.
+// fy != 0 adds the Poiseuille body force along v_y (:429-437, :458-466):
+// fc -= fy (f[j+1] - f[j-1]), with -fy f[1] at j = 0 and -fy f[N-2] at
+// j = N-1 as the reference writes them; fy = Ma dt / (4 h_v).
 __global__ void __launch_bounds__(ET) k_upwind2(int N, int nX, const double *__restrict__ f, double *__restrict__ fc,
                                                 const double *__restrict__ fl, const double *__restrict__ fr,
                                                 const double *__restrict__ v, const double *__restrict__ x,
-                                                const double *__restrict__ dx, double dt) {
+                                                const double *__restrict__ dx, double dt, double fy) {
   const size_t n3 = (size_t)N * N * N, total = (size_t)nX * n3;
   const size_t e = (size_t)blockIdx.x * ET + threadIdx.x;
   if (e >= total) return;
@@ -160,6 +165,13 @@ __global__ void __launch_bounds__(ET) k_upwind2(int N, int nX, const double *__r
       const double s2 = slope(f, n3, t, l + 1, x);
       r = f0 - cfl * (f[(size_t)(l + 1) * n3 + t] - 0.5 * dx[l + 1] * s2 - (f0 - 0.5 * dx[l] * s1));
     }
+  }
+  if (fy != 0.0) {
+    const int j = (int)((t / N) % N);
+    const double *fj = f + (size_t)l * n3 + t;
+    if (j == 0) r = r - fy * fj[N];
+    else if (j == N - 1) r = r - fy * fj[-N];
+    else r = r - fy * (fj[N] - fj[-N]);
   }
   fc[(size_t)l * n3 + t] = r;
 }
@@ -216,9 +228,10 @@ extern "C" int spbte_launch_walls2(int N, int nX, int diffuse, const double *f, 
 }
 
 extern "C" int spbte_launch_upwind2(int N, int nX, const double *f, double *fc, const double *fl, const double *fr,
-                                    const double *v, const double *x, const double *dx, double dt, hipStream_t s) {
+                                    const double *v, const double *x, const double *dx, double dt, double fy,
+                                    hipStream_t s) {
   const size_t n = (size_t)nX * N * N * N;
-  hipLaunchKernelGGL(k_upwind2, dim3(blocks(n)), dim3(ET), 0, s, N, nX, f, fc, fl, fr, v, x, dx, dt);
+  hipLaunchKernelGGL(k_upwind2, dim3(blocks(n)), dim3(ET), 0, s, N, nX, f, fc, fl, fr, v, x, dx, dt, fy);
   return (int)hipGetLastError();
 }
 

@@ -461,6 +461,27 @@ int spbte_op_get_weights_rows(spbte_op *op, double **rows) {
   return SPBTE_OK;
 }
 
+int spbte_op_store_weights_file(spbte_op *op, const char *path) {
+  if (!op || !path) return fail(SPBTE_ERR_ARG, "null argument");
+  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  HIPCHK(hipStreamSynchronize(op->s));
+  const size_t n3 = op->n3, stage_rows = (((size_t)64 << 20) / (n3 * 8)) ? ((size_t)64 << 20) / (n3 * 8) : 1;
+  double *stage = NULL;
+  if (hipHostMalloc((void **)&stage, sizeof(double) * n3 * stage_rows, hipHostMallocDefault) != hipSuccess)
+    return fail(SPBTE_ERR_HIP, "pinned staging allocation failed");
+  FILE *fp = fopen(path, "wb");
+  int rc = fp ? SPBTE_OK : fail(SPBTE_ERR_IO, "cannot open %s for writing", path);
+  for (size_t r0 = 0; r0 < n3 && !rc; r0 += stage_rows) {
+    const size_t nr = (r0 + stage_rows <= n3) ? stage_rows : n3 - r0;
+    hipError_t e = hipMemcpy(stage, op->dW64 + r0 * n3, sizeof(double) * n3 * nr, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(SPBTE_ERR_HIP, "weight download: %s", hipGetErrorString(e));
+    else if (fwrite(stage, sizeof(double), n3 * nr, fp) != n3 * nr) rc = fail(SPBTE_ERR_IO, "writing %s failed", path);
+  }
+  if (fp && fclose(fp) != 0 && !rc) rc = fail(SPBTE_ERR_IO, "writing %s failed", path);
+  hipHostFree(stage);
+  return rc;
+}
+
 /* Storage of the table each form streams: FP64, FP32 (forms 1 and 3) or FP16x2. */
 static int storage_of(int form) { return form == SPBTE_FORM_FP32A ? SPBTE_FORM_FP32 : form; }
 

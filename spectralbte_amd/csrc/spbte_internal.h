@@ -91,7 +91,7 @@ int spbte_launch_diffuse(const spbte_grid_dev *g, int nwall, const double *const
 int spbte_launch_walls2(int N, int nX, int diffuse, const double *f, double *fl, double *fr, const double *x,
                         const double *dx, hipStream_t s);
 int spbte_launch_upwind2(int N, int nX, const double *f, double *fc, const double *fl, const double *fr,
-                         const double *v, const double *x, const double *dx, double dt, hipStream_t s);
+                         const double *v, const double *x, const double *dx, double dt, double fy, hipStream_t s);
 int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const double *y, double c, const double *z,
                          double *out, hipStream_t s);
 int spbte_launch_out_quantities(const spbte_grid_dev *g, const double *f, int ncell, double *out, hipStream_t s);

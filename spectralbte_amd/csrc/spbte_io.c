@@ -79,7 +79,7 @@ int spbte_run_open_output(spbte_run *run, const char *path, int flags, int appen
   if (rc) return rc;
   FILE *fp = fopen(path, append ? "a" : "w");
   if (!fp) return spbte_fail(SPBTE_ERR_IO, "cannot open %s", path);
-  if (!append) {
+  {  /* the header is written on append too, as src/output.c:100-135 does */
     fprintf(fp, "#Time ");
     if (dim == 1) fprintf(fp, "Position ");
     if (flags & SPBTE_OUT_DENSITY) fprintf(fp, "Density ");
@@ -134,8 +134,11 @@ int spbte_run_write_output(spbte_run *run, double time) {
   }
   if (!rc && dim == 1 && hipMemcpy(x, d_x, sizeof(double) * ncell, hipMemcpyDeviceToHost) != hipSuccess)
     rc = spbte_fail(SPBTE_ERR_HIP, "mesh copy failed");
+  /* species mass m: rho = m int f, u unchanged, T = (m / KB) times the
+   * mass-1 temperature (src/momentRoutines.c:58-183); exact for m = KB = 1 */
+  const double m = g->mass, tfac = g->mass / g->kb;
   for (int c = 0; c < ncell && !rc; c++) {
-    const double rho = mom[5 * c], ux = mom[5 * c + 1], T = mom[5 * c + 4];
+    const double rho = m * mom[5 * c], ux = mom[5 * c + 1], T = tfac * mom[5 * c + 4];
     FILE *fp = sl->fp;
     fprintf(fp, "%le ", time);
     if (dim == 1) fprintf(fp, "%le ", x[c]);

@@ -9,6 +9,7 @@
  * evaluation per collision stage, elementwise updates), and it can be captured
  * once as a HIP graph and replayed.  The state never leaves HBM between steps.
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -45,6 +46,7 @@ struct spbte_run {
   double dt, Kn;
   int bc;       /* 1D walls: SPBTE_BC_* */
   double T0, T1;
+  double force_ma; /* Poiseuille body force along v_y (order 2), 0 = none */
   size_t n3;
   int nghost;   /* ghost cells per side (1D: order; 0D: 0) */
   double *d_x, *d_dx;            /* 1D: nX + 2 order */
@@ -124,7 +126,7 @@ int spbte_run0d_create(spbte_run **out, spbte_op *op, int ncell, int order, doub
 int spbte_run1d_create(spbte_run **out, spbte_op *op, int nX, int order, const double *x, const double *dx, double dt,
                        double Kn, int bc, double T0, double T1, int form) {
   if (!x || !dx) return spbte_fail(SPBTE_ERR_ARG, "x and dx are required");
-  if (bc < SPBTE_BC_COPY || bc > SPBTE_BC_PERIODIC) return spbte_fail(SPBTE_ERR_ARG, "bad boundary condition %d", bc);
+  if (bc < SPBTE_BC_COPY || bc > SPBTE_BC_DIFFUSE_LEFT) return spbte_fail(SPBTE_ERR_ARG, "bad boundary condition %d", bc);
   if (order == 2 && bc == SPBTE_BC_PERIODIC)
     return spbte_fail(SPBTE_ERR_UNSUPPORTED, "periodic ghosts exist for order 1 only (src/transportroutines.c:156-172)");
   if (order == 2 && nX < 2) return spbte_fail(SPBTE_ERR_ARG, "order 2 needs nX >= 2");
@@ -198,7 +200,8 @@ int spbte_run_moments(spbte_run *r, double *out) {
 static int upwind_two(spbte_run *r, const double *d_x, const double *d_dx, double *src, double *dst) {
   const spbte_grid_dev *g = spbte_op_grid(r->op);
   hipStream_t s = spbte_op_stream(r->op);
-  const int N = g->N, nX = r->ncell, diffuse = r->bc == SPBTE_BC_DIFFUSE;
+  const int N = g->N, nX = r->ncell;
+  const int diffuse = r->bc == SPBTE_BC_DIFFUSE ? 3 : r->bc == SPBTE_BC_DIFFUSE_LEFT ? 1 : 0;
   LCHK(spbte_launch_ghosts(N, nX, 3, src, s));
   LCHK(spbte_launch_walls2(N, nX, diffuse, src, r->d_fl, r->d_fr, d_x, d_dx, s));
   if (diffuse) {
@@ -206,9 +209,11 @@ static int upwind_two(spbte_run *r, const double *d_x, const double *d_dx, doubl
     double *wd[2] = {r->d_fl, r->d_fr};
     const double TW[2] = {r->T0, r->T1};
     const int right[2] = {0, 1};
-    LCHK(spbte_launch_diffuse(g, 2, ws, wd, TW, right, s));
+    LCHK(spbte_launch_diffuse(g, diffuse == 3 ? 2 : 1, ws, wd, TW, right, s));
   }
-  LCHK(spbte_launch_upwind2(N, nX, src, dst, r->d_fl, r->d_fr, (const double *)g->d_v, d_x, d_dx, r->dt, s));
+  /* Ma*0.5*dt/(2*h_v) with h_v = 2 L_v / (N - 1), src/transportroutines.c:37,429 */
+  const double fy = r->force_ma != 0.0 ? r->force_ma * 0.5 * r->dt / (2 * (2 * g->Lv / (N - 1))) : 0.0;
+  LCHK(spbte_launch_upwind2(N, nX, src, dst, r->d_fl, r->d_fr, (const double *)g->d_v, d_x, d_dx, r->dt, fy, s));
   return SPBTE_OK;
 }
 
@@ -225,6 +230,14 @@ static int advect(spbte_run *r, double *src, double *dst) {
       const double TW[2] = {r->T0, r->T1};
       const int right[2] = {0, 1};
       LCHK(spbte_launch_diffuse(g, 2, ws, wd, TW, right, s));
+    } else if (r->bc == SPBTE_BC_DIFFUSE_LEFT) {
+      /* right ghost copies cell nX; the left ghost's incoming half is the wall's */
+      const double *ws[1] = {src + r->n3};
+      double *wd[1] = {src};
+      const double TW[1] = {r->T0};
+      const int right[1] = {0};
+      LCHK(spbte_launch_ghosts(N, nX, 0, src, s));
+      LCHK(spbte_launch_diffuse(g, 1, ws, wd, TW, right, s));
     } else {
       LCHK(spbte_launch_ghosts(N, nX, r->bc == SPBTE_BC_PERIODIC ? 2 : 0, src, s));
     }
@@ -250,6 +263,18 @@ static int collide(spbte_run *r, const double *d_in) {
   r->s_exact += n_exact;
   if (n_exact == r->ncell && r->check_every > 1) r->s_fallbacks++;
   if (!(err <= r->s_max_err)) r->s_max_err = err;
+  return SPBTE_OK;
+}
+
+int spbte_run_set_forcing(spbte_run *r, double Ma) {
+  if (!r || !isfinite(Ma)) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
+  if (r->dim != 1 || r->order != 2)
+    return spbte_fail(SPBTE_ERR_UNSUPPORTED,
+                      "the v_y body force exists for 1D order-2 runs only: the reference's order-1 term is applied "
+                      "after its k loop and writes outside the cell (src/transportroutines.c:218-226)");
+  HIPCHK(hipStreamSynchronize(spbte_op_stream(r->op)));
+  drop_graph(r);
+  r->force_ma = Ma;
   return SPBTE_OK;
 }
 

@@ -164,6 +164,36 @@ def test_run1d_order2_vs_oracle(spb, orc, bc):
     op.close()
 
 
+@pytest.mark.parametrize("order,bc,Ma", [(1, 3, 0.0), (2, 3, 0.0), (2, 1, 1.0), (2, 0, 1.0)])
+def test_run1d_left_wall_and_body_force_vs_oracle(spb, orc, order, bc, Ma):
+    """The diffuse left wall at 2 TWall of ICChoice 1 (BC_DIFFUSE_LEFT) and the
+    order-2 Poiseuille body force of ICChoice 5 against the oracle's run, with
+    the graph path on."""
+    N, Lv, nX, dt, Kn, steps = 8, 5.0, 10, 0.002, 0.1, 3
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    W = op.get_weights()
+    x, dx = orc.uniform_mesh(nX, 1.0, order)
+    f0 = _shock_cells(orc, v, nX)
+    run = spb.Run.mesh(op, f0, x, dx, order=order, dt=dt, Kn=Kn, bc=bc, T0=2.0, T1=2.0)
+    if Ma:
+        run.set_forcing(Ma)
+    run.use_graph(True)
+    states = []
+    run_cb = lambda t, f: states.append(f[order:nX + order].copy())  # noqa: E731
+    orc.run_1d(N, v, eta, W, f0, x, dx, dt, steps, order, Kn, bc, T0=2.0, T1=2.0, callback=run_cb, Ma=Ma)
+    for t in range(steps):
+        run.step(1)
+        g = run.state()
+        assert np.abs(g - states[t]).max() < 1e-12 * np.abs(states[t]).max(), t
+    if order == 1:
+        with pytest.raises(spb.SpbteError, match="order-2"):
+            run.set_forcing(1.0)
+    run.close()
+    op.close()
+
+
 @pytest.mark.parametrize("dim", [0, 1])
 def test_graph_replays_match_eager(spb, orc, dim):
     """A captured step replayed several times, with eager work and synchronisations

@@ -227,3 +227,50 @@ def test_transport_convergence_orders(orc):
     assert np.all((r1 > 0.8) & (r1 < 1.2)), r1
     assert np.all(r2 > 1.5), r2
     assert e2[-1] < 0.2 * e1[-1]
+
+
+def test_left_diffuse_wall_composes_diffuse_and_copy(orc):
+    """bc 3 (ICChoice 1: diffuse left wall, copied right ghost) equals the
+    diffuse-wall result away from the right wall and the copy result away from
+    the left wall, at order 1 and order 2."""
+    N, Lv, nX, dt = 8, 5.0, 6, 0.002
+    v, _ = orc.grid_1d(N, Lv)
+    rng = np.random.default_rng(5)
+    for order in (1, 2):
+        x, dx = orc.uniform_mesh(nX, 1.0, order)
+        f = np.zeros((nX + 2 * order, N ** 3))
+        for c in range(nX):
+            f[order + c] = orc.maxwellian_1d(v, 1 + 0.2 * rng.random(), 0.3 * rng.random(), 1 + rng.random()).ravel()
+        if order == 1:
+            run = lambda bc: orc.upwind_one(N, nX, v, dx, dt, bc, f, T0=2.0)[1]  # noqa: E731
+        else:
+            run = lambda bc: orc.upwind_two(N, nX, v, x, dx, dt, bc, f, T0=2.0)[1]  # noqa: E731
+        mixed, diffuse, copy = run(3), run(1), run(0)
+        lo, hi = order, order + nX
+        assert np.array_equal(mixed[lo:hi - 1], diffuse[lo:hi - 1])
+        assert np.array_equal(mixed[lo + 1:hi], copy[lo + 1:hi])
+        assert not np.array_equal(mixed[lo], copy[lo]) and not np.array_equal(mixed[hi - 1], diffuse[hi - 1])
+
+
+def test_poiseuille_force_adds_y_momentum(orc):
+    """The order-2 body force of ICChoice 5 (src/transportroutines.c:429-437,
+    :458-466) is -Ma dt/2 d f/d v_y per stage: on a smooth Maxwellian it adds
+    Ma dt/2 rho to the y momentum (integration by parts) and leaves density,
+    x momentum and the z direction alone, to the quadrature error."""
+    N, Lv, nX, dt, Ma = 24, 6.0, 4, 0.01, 1.0
+    v, _ = orc.grid_1d(N, Lv)
+    x, dx = orc.uniform_mesh(nX, 1.0, 2)
+    f = np.tile(orc.maxwellian_1d(v, 1.0, 0.0, 1.0).ravel(), (nX + 4, 1))
+    base = orc.upwind_two(N, nX, v, x, dx, dt, 0, f)[1]
+    forced = orc.upwind_two(N, nX, v, x, dx, dt, 0, f, Ma=Ma)[1]
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    h3 = (v[1] - v[0]) ** 3
+    W3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).ravel() * h3
+    VX = np.repeat(v, N * N)
+    VY = np.tile(np.repeat(v, N), N)
+    VZ = np.tile(v, N * N)
+    rho = W3 @ f[3]
+    for l in range(2, nX + 2):
+        d = forced[l] - base[l]
+        assert abs(W3 @ (VY * d) - Ma * 0.5 * dt * rho) < 1e-3 * Ma * 0.5 * dt * rho
+        assert abs(W3 @ d) < 1e-9 and abs(W3 @ (VX * d)) < 1e-9 and abs(W3 @ (VZ * d)) < 1e-9
