@@ -85,6 +85,12 @@ int spbte_op_set_weights(spbte_op *op, const double *W);
 int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows);
 int spbte_op_load_weights_file(spbte_op *op, const char *path);
 int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, double d_j, double m_i, double m_j);
+/* The anisotropic generator's table with glance = 0, the Landau-Fokker-Planck
+ * limit: gHat3L (exec/MPIcollisionRoutines.c:205-281, src/aniso_weights.c:352-428)
+ * at every (row, column) pair of the operator's eta grid and L_v, by nested
+ * 64-point Gauss-Legendre rules, on the GPU.  As the reference writes it, the
+ * table carries no trapezoid or h_eta^3 factors. */
+int spbte_op_generate_weights_landau(spbte_op *op, double lambda);
 int spbte_op_get_weights(spbte_op *op, double *W);
 /* Writes the FP64 table as a Weights/ file (N^3 rows of N^3 doubles, the
  * layout src/weights.c:95-104 stores). */

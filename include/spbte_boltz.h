@@ -76,6 +76,11 @@ int spbte_initial_state(const spbte_input *in, const species *sp, const double *
  * messages go to stdout, errors to stderr. */
 int spbte_boltz_main(int argc, char **argv);
 
+/* exec/MPIWeightGenerator.c main on one GPU: argv = N glance beta lambda L_v.
+ * glance 0 writes Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts (kept if it
+ * exists); glance > 0 is refused (not built).  Returns 0 on success. */
+int spbte_weightgen_main(int argc, char **argv);
+
 #ifdef __cplusplus
 }
 #endif

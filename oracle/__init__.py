@@ -59,6 +59,9 @@ def lib():
             "orc_conserve": (None, [I, _dp, I, _dp, _dp, _ip, _dp]),
             "orc_ghat_integral": (D, [D, D, D, D, D]),
             "orc_weights_iso": (None, [I, _dp, D, D, D, D, D, D, _dp, I]),
+            "orc_gauss_legendre64": (None, [_dp, _dp]),
+            "orc_landau_entry": (D, [D, D, D, D, D, D, D, D]),
+            "orc_weights_landau_rows": (None, [I, _dp, D, D, ctypes.c_long, ctypes.c_long, _dp]),
             "orc_weights_load": (I, [ctypes.c_char_p, I, _dp]),
             "orc_diffuse_bc": (None, [I, _dp, D, _dp, _dp, D, I]),
             "orc_upwind_one": (None, [I, I, _dp, D, _dp, D, I, D, D, _dp, _dp]),
@@ -184,6 +187,26 @@ def ghat_integral(a0, a1, a2, Lv, lam):
 def weights_iso(N, eta, Lv, lam, d_i=2.0, d_j=2.0, m_i=1.0, m_j=1.0, nthreads=None):
     W = np.zeros((N ** 3, N ** 3))
     lib().orc_weights_iso(N, _p(_c(eta)), Lv, lam, d_i, d_j, m_i, m_j, _p(W), nthreads or default_threads())
+    return W
+
+
+def gauss_legendre64():
+    """(x, w): the 32 positive nodes (ascending) and weights of the 64-point rule."""
+    x, w = np.zeros(32), np.zeros(32)
+    lib().orc_gauss_legendre64(_p(x), _p(w))
+    return x, w
+
+
+def landau_entry(zeta, xi, Lv, lam):
+    """gHat3L (exec/MPIcollisionRoutines.c:219-248) for row mode zeta, column mode xi."""
+    return lib().orc_landau_entry(*map(float, zeta), *map(float, xi), float(Lv), float(lam))
+
+
+def weights_landau(N, eta, Lv, lam, rows=None):
+    """Rows of the glance = 0 (Landau) table; rows = (r0, nrows) or all."""
+    r0, nr = rows if rows is not None else (0, N ** 3)
+    W = np.zeros((nr, N ** 3))
+    lib().orc_weights_landau_rows(N, _p(_c(eta)), Lv, lam, r0, nr, _p(W))
     return W
 
 

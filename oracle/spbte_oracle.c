@@ -540,6 +540,96 @@ void orc_weights_iso(int N, const double *eta, double Lv, double lam, double d_i
   }
 }
 
+/* 64-point Gauss-Legendre rule on [-1, 1]: the 32 positive nodes in
+ * ascending order and their weights, from Newton's iteration on P_64 in long
+ * double (the tables gsl_integration_glfixed and gauss_legendre use). */
+void orc_gauss_legendre64(double *x, double *w) {
+  const int n = 64;
+  for (int i = 0; i < n / 2; i++) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L)), pp = 0.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p1 = 1.0L, p2 = 0.0L;
+      for (int j = 1; j <= n; j++) {
+        long double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0L);
+      long double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) < 1e-19L) break;
+    }
+    /* i = 0 is the largest root; store ascending */
+    x[n / 2 - 1 - i] = (double)z;
+    w[n / 2 - 1 - i] = (double)(2.0L / ((1.0L - z * z) * pp * pp));
+  }
+}
+
+/* gsl_integration_glfixed / gauss_legendre for even n: A = (b-a)/2,
+ * B = (b+a)/2, A * sum_i w_i (f(B + A x_i) + f(B - A x_i)). */
+typedef double (*orc_fn)(double, const double *);
+static double gl64_integrate(orc_fn f, const double *args, double a, double b, const double *x, const double *w) {
+  double A = 0.5 * (b - a), B = 0.5 * (b + a), s = 0.0;
+  for (int i = 0; i < 32; i++) {
+    double Ax = A * x[i];
+    s += w[i] * (f(B + Ax, args) + f(B - Ax, args));
+  }
+  return A * s;
+}
+
+/* ghatL2 (exec/MPIcollisionRoutines.c:205-210, src/aniso_weights.c:352-357);
+ * args = {xiperp, |zeta|^2, xi.zeta/|zeta|, |zeta|, r}. */
+static double landau_theta(double th, const double *a) {
+  double r = a[4];
+  return sin(th) * j0(r * a[0] * sin(th)) *
+         (-r * r * a[1] * sin(th) * sin(th) * cos(r * a[2] * cos(th)) + 4 * r * a[3] * sin(r * a[2] * cos(th)) * cos(th));
+}
+
+static double orc_landau_lam;
+static const double *orc_glx, *orc_glw;
+
+/* ghatL (:212-217, :359-369): r^(lam+2) * int_0^pi ghatL2 dtheta */
+static double landau_r(double r, const double *a) {
+  double b[5] = {a[0], a[1], a[2], a[3], r};
+  return pow(r, orc_landau_lam + 2) * gl64_integrate(landau_theta, b, 0, M_PI, orc_glx, orc_glw);
+}
+
+/* gHat3L (:219-248, :371-398) for row mode zeta and column mode xi. */
+double orc_landau_entry(double z1, double z2, double z3, double x1, double x2, double x3, double Lv, double lam) {
+  double gx[32], gw[32], a[4];
+  orc_gauss_legendre64(gx, gw);
+  double zetalen2 = z3 * z3 + z2 * z2 + z1 * z1;
+  double xilen2 = x1 * x1 + x2 * x2 + x3 * x3;
+  double xizeta = x1 * z1 + x2 * z2 + x3 * z3;
+  double zetalen = sqrt(zetalen2), xiperp;
+  if (((xilen2 - xizeta * xizeta / zetalen2) < 0) || (zetalen2 == 0)) xiperp = 0;
+  else xiperp = sqrt(xilen2 - xizeta * xizeta / zetalen2);
+  a[0] = xiperp;
+  a[1] = zetalen2;
+  a[2] = (zetalen != 0) ? xizeta / zetalen : 0.0;
+  a[3] = zetalen;
+  orc_landau_lam = lam;
+  orc_glx = gx;
+  orc_glw = gw;
+  return 2.0 * M_PI * gl64_integrate(landau_r, a, 0, Lv, gx, gw);
+}
+
+/* generate_conv_weights with glance = 0 (exec/MPIcollisionRoutines.c:251-281,
+ * src/aniso_weights.c:400-428): W[r][c] = gHat3L(eta_r, eta_c) for rows
+ * r0 .. r0+nrows-1 (no trapezoid or h_eta^3 factors, as the reference writes
+ * the table).  W holds nrows x N^3.  Single-threaded (the integrand keeps its
+ * rule in statics). */
+void orc_weights_landau_rows(int N, const double *eta, double Lv, double lam, long r0, long nrows, double *W) {
+  long n3 = (long)N * N * N;
+  for (long r = r0; r < r0 + nrows; r++) {
+    int i = (int)(r / ((long)N * N)), j = (int)((r / N) % N), k = (int)(r % N);
+    for (long c = 0; c < n3; c++) {
+      int l = (int)(c / ((long)N * N)), m = (int)((c / N) % N), nn = (int)(c % N);
+      W[(r - r0) * n3 + c] = orc_landau_entry(eta[i], eta[j], eta[k], eta[l], eta[m], eta[nn], Lv, lam);
+    }
+  }
+}
+
 /* src/weights.c:68,78-88: raw native-endian FP64, N^3 rows of N^3, no header.
  * Returns 0 on success, -1 if the file is missing, -2 if it is short. */
 int orc_weights_load(const char *path, int N, double *W) {

@@ -142,6 +142,8 @@ def lib():
                                               ctypes.POINTER(ctypes.c_long), _dp]),
             "spbte_run_set_forcing": (I, [_vp, D]),
             "spbte_op_store_weights_file": (I, [_vp, ctypes.c_char_p]),
+            "spbte_op_generate_weights_landau": (I, [_vp, D]),
+            "spbte_weightgen_main": (I, [I, ctypes.POINTER(ctypes.c_char_p)]),
             # front end (spbte_boltz.h)
             "spbte_read_input": (I, [ctypes.c_char_p, _vp]),
             "spbte_species_defaults": (None, [_vp, I]),
@@ -257,6 +259,10 @@ class Operator:
 
     def generate_weights_iso(self, lam, d_i=2.0, d_j=2.0, m_i=1.0, m_j=1.0):
         _check(lib().spbte_op_generate_weights_iso(self.h, lam, d_i, d_j, m_i, m_j))
+
+    def generate_weights_landau(self, lam):
+        """The anisotropic generator's glance = 0 (Landau) table on this grid."""
+        _check(lib().spbte_op_generate_weights_landau(self.h, float(lam)))
 
     def store_weights_file(self, path):
         _check(lib().spbte_op_store_weights_file(self.h, os.fsencode(path)))
@@ -774,6 +780,21 @@ def boltz(input_name, flags_name, workdir="."):
     os.chdir(workdir)
     try:
         rc = lib().spbte_boltz_main(3, argv)
+    finally:
+        os.chdir(prev)
+    if rc != 0:
+        raise SpbteError(lib().spbte_last_error().decode())
+
+
+def weightgen(N, glance, beta, lam, L_v, workdir="."):
+    """Runs spbte_weightgen_main (exec/MPIWeightGenerator.c) in workdir, which
+    holds Weights/.  Raises on failure."""
+    prev = os.getcwd()
+    args = [b"weightgen_amd"] + [str(a).encode() for a in (N, glance, beta, lam, L_v)]
+    argv = (ctypes.c_char_p * len(args))(*args)
+    os.chdir(workdir)
+    try:
+        rc = lib().spbte_weightgen_main(len(args), argv)
     finally:
         os.chdir(prev)
     if rc != 0:

@@ -1,6 +1,7 @@
-// k_weights.hip -- isotropic convolution weights generated on the GPU
-// (reference: src/weights.c:265-281 generate_conv_weights_iso with gHat3,
-// :181-262, and the integrand ghat, :156-160).
+// k_weights.hip -- convolution weight tables generated on the GPU: the
+// isotropic table (reference: src/weights.c:265-281 generate_conv_weights_iso
+// with gHat3, :181-262, and the integrand ghat, :156-160) and the Landau
+// table of the anisotropic generator with glance = 0 (further below).
 //
 //   W[r][c] = w_c1 w_c2 w_c3 * dfac * P * I_lam(mu|eta_r|, |eta_c|, |eta_c - mu eta_r|)
 //   I_lam(a0,a1,a2) = int_0^L r^(lam+2) [sinc(a0 r) sinc(a2 r) - sinc(a1 r)] dr
@@ -122,7 +123,117 @@ void gauss_legendre20(GL20 &q) {
   }
 }
 
+// ---- Landau table (glance = 0): gHat3L of exec/MPIcollisionRoutines.c:205-248
+// (src/aniso_weights.c:352-398), one thread per entry:
+//   W[r][c] = 2 pi int_0^L rho^(lam+2) int_0^pi g(theta) dtheta drho,
+//   g = sin(t) J0(rho a0 sin t) (-rho^2 a1 sin^2 t cos(rho a2 cos t) + 4 rho a3 sin(rho a2 cos t) cos t),
+//   a0 = |xi_perp|, a1 = |zeta|^2, a2 = xi.zeta/|zeta|, a3 = |zeta| (zeta = row mode, xi = column mode),
+// both integrals by the 64-point Gauss-Legendre rule summed in symmetric node
+// pairs, as gsl_integration_glfixed and gauss_legendre do.  The node
+// abscissae, their sines and cosines and rho^(lam+2) are the same for every
+// entry and are evaluated once on the host with the C library.
+struct LandauRule {
+  double w[32];                    // weights of the positive nodes, ascending nodes
+  double st_p[32], ct_p[32];       // sin, cos of theta = B + A x_k on [0, pi]
+  double st_m[32], ct_m[32];       // sin, cos of theta = B - A x_k
+  double r_p[32], r_m[32];         // rho = B + A x_i, B - A x_i on [0, L]
+  double rl_p[32], rl_m[32];       // rho^(lam+2)
+  double At, Ar;                   // half-lengths of the two intervals
+};
+
+__device__ __forceinline__ double landau_g(double r, double a0, double a1, double a2, double a3, double st, double ct) {
+  double sv, cv;
+  sincos(r * a2 * ct, &sv, &cv);
+  return st * j0(r * a0 * st) * (-r * r * a1 * st * st * cv + 4 * r * a3 * sv * ct);
+}
+
+__global__ void __launch_bounds__(256) k_weights_landau(int N, const double *__restrict__ eta, long r0, long nrows,
+                                                        LandauRule q, double *__restrict__ W) {
+  const long n3 = (long)N * N * N;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nrows * n3) return;
+  const long r = r0 + e / n3, c = e % n3;
+  const double z1 = eta[r / ((long)N * N)], z2 = eta[(r / N) % N], z3 = eta[r % N];
+  const double x1 = eta[c / ((long)N * N)], x2 = eta[(c / N) % N], x3 = eta[c % N];
+  const double zetalen2 = z3 * z3 + z2 * z2 + z1 * z1;
+  const double xilen2 = x1 * x1 + x2 * x2 + x3 * x3;
+  const double xizeta = x1 * z1 + x2 * z2 + x3 * z3;
+  const double zetalen = sqrt(zetalen2);
+  const double perp2 = xilen2 - xizeta * xizeta / zetalen2;
+  const double a0 = (perp2 < 0 || zetalen2 == 0) ? 0.0 : sqrt(perp2);
+  const double a2 = zetalen != 0 ? xizeta / zetalen : 0.0;
+  double sr = 0.0;
+  for (int i = 0; i < 32; ++i) {
+    double f[2];
+    for (int h = 0; h < 2; ++h) {
+      const double rho = h ? q.r_m[i] : q.r_p[i];
+      double st = 0.0;
+      for (int k = 0; k < 32; ++k)
+        st += q.w[k] * (landau_g(rho, a0, zetalen2, a2, zetalen, q.st_p[k], q.ct_p[k]) +
+                        landau_g(rho, a0, zetalen2, a2, zetalen, q.st_m[k], q.ct_m[k]));
+      f[h] = (h ? q.rl_m[i] : q.rl_p[i]) * (q.At * st);
+    }
+    sr += q.w[i] * (f[0] + f[1]);
+  }
+  W[(r - r0) * n3 + c] = 2.0 * M_PI * (q.Ar * sr);
+}
+
+// n-point Gauss-Legendre rule: the n/2 positive nodes ascending and their
+// weights, Newton's iteration on P_n in long double.
+void gauss_legendre_half(int n, double *x, double *w) {
+  for (int i = 0; i < n / 2; i++) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L)), pp = 0.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p1 = 1.0L, p2 = 0.0L;
+      for (int j = 1; j <= n; j++) {
+        const long double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0L);
+      const long double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) < 1e-19L) break;
+    }
+    x[n / 2 - 1 - i] = (double)z;
+    w[n / 2 - 1 - i] = (double)(2.0L / ((1.0L - z * z) * pp * pp));
+  }
+}
+
 }  // namespace
+
+extern "C" int spbte_launch_weights_landau(const spbte_grid_dev *g, const double *eta, double lam, double *W,
+                                           hipStream_t s) {
+  const int N = g->N;
+  const long n3 = (long)N * N * N;
+  LandauRule q;
+  double x[32];
+  gauss_legendre_half(64, x, q.w);
+  q.At = 0.5 * (M_PI - 0.0);
+  q.Ar = 0.5 * (g->Lv - 0.0);
+  const double Bt = 0.5 * (M_PI + 0.0), Br = 0.5 * (g->Lv + 0.0);
+  for (int k = 0; k < 32; k++) {
+    const double At_x = q.At * x[k], Ar_x = q.Ar * x[k];
+    q.st_p[k] = sin(Bt + At_x);
+    q.ct_p[k] = cos(Bt + At_x);
+    q.st_m[k] = sin(Bt - At_x);
+    q.ct_m[k] = cos(Bt - At_x);
+    q.r_p[k] = Br + Ar_x;
+    q.r_m[k] = Br - Ar_x;
+    q.rl_p[k] = pow(q.r_p[k], lam + 2);
+    q.rl_m[k] = pow(q.r_m[k], lam + 2);
+  }
+  // rows per launch: about 8M entries (a fraction of a second each at N = 32)
+  const long rows = n3 >= (8L << 20) ? 1 : (8L << 20) / n3;
+  for (long r0 = 0; r0 < n3; r0 += rows) {
+    const long nr = r0 + rows <= n3 ? rows : n3 - r0;
+    const unsigned blocks = (unsigned)((nr * n3 + 255) / 256);
+    hipLaunchKernelGGL(k_weights_landau, dim3(blocks), dim3(256), 0, s, N, eta, r0, nr, q, W + r0 * n3);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+  }
+  return 0;
+}
 
 extern "C" int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
                                         double *W, hipStream_t s) {

@@ -369,9 +369,10 @@ static int walls_of(int init, int order, double *T0, double *T1) {
   return SPBTE_BC_COPY;
 }
 
-/* The weight table as exec/boltz.c:149-163 obtains it: isotropic tables from
- * Weights/ (src/weights.c:65-73) or generated and stored; anisotropic (Landau)
- * tables from Weights/ only (src/aniso_weights.c:65-81). */
+/* The weight table as exec/boltz.c:149-163 obtains it, from Weights/ or
+ * generated on the GPU and stored there: isotropic (src/weights.c:65-124) or,
+ * for Anisotropic 1, the glance = 0 Landau table (src/aniso_weights.c:37-146,
+ * which boltz_ calls with glance 0). */
 static int setup_weights(spbte_op *op, const spbte_input *in, const species *sp) {
   char name[512];
   if (in->aniso)
@@ -392,11 +393,63 @@ static int setup_weights(spbte_op *op, const spbte_input *in, const species *sp)
   } else {
     printf("Fresh version of weights being computed and stored for this configuration\n");
   }
-  if (in->aniso)
-    return spbte_fail(SPBTE_ERR_UNSUPPORTED,
-                      "anisotropic weights %s must be generated beforehand (the reference's weight generator)", name);
-  CHK(spbte_op_generate_weights_iso(op, in->lambda, sp->d_ref, sp->d_ref, sp->mass, sp->mass));
+  if (in->aniso) CHK(spbte_op_generate_weights_landau(op, in->lambda));
+  else CHK(spbte_op_generate_weights_iso(op, in->lambda, sp->d_ref, sp->d_ref, sp->mass, sp->mass));
   return spbte_op_store_weights_file(op, name);
+}
+
+/* exec/MPIWeightGenerator.c main on one GPU: N glance beta lambda L_v.  The
+ * grid is the generator's own (h_eta = 2 pi / (N h_v), L_eta = N h_eta / 2,
+ * :63-74); an existing file is kept (:87-90).  glance = 0 writes the Landau
+ * table; the glancing-collision table (glance > 0, nested adaptive GSL
+ * quadrature, exec/MPIcollisionRoutines.c:33-188) is not built. */
+int spbte_weightgen_main(int argc, char **argv) {
+  if (argc < 6 || !argv) {
+    fprintf(stderr, "usage: %s N glance beta lambda L_v   (writes Weights/N<N>_Aniso_L_v<L_v>_lambda<lambda>_Landau.wts)\n",
+            (argc > 0 && argv && argv[0]) ? argv[0] : "weightgen_amd");
+    return 1;
+  }
+  const int N = atoi(argv[1]);
+  const double glance = atof(argv[2]), lambda = atof(argv[4]), L_v = atof(argv[5]);
+  int rc = SPBTE_OK;
+  if (N < 2 || !(L_v > 0) || !isfinite(lambda))
+    rc = spbte_fail(SPBTE_ERR_ARG, "need N >= 2, L_v > 0 and a finite lambda");
+  else if (glance != 0)
+    rc = spbte_fail(SPBTE_ERR_UNSUPPORTED,
+                    "glance %g: only the Landau table (glance 0) is generated here; the glancing-collision table uses "
+                    "nested adaptive quadrature (exec/MPIcollisionRoutines.c:33-188) and is not built",
+                    glance);
+  char name[512];
+  snprintf(name, sizeof name, "Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts", N, L_v, lambda);
+  FILE *fp = rc ? NULL : fopen(name, "rb");
+  if (fp) {
+    fclose(fp);
+    printf("Stored weights already computed! \n");
+    return 0;
+  }
+  double *v = rc ? NULL : malloc(sizeof(double) * N), *eta = rc ? NULL : malloc(sizeof(double) * N);
+  spbte_op *op = NULL;
+  if (!rc && (!v || !eta)) rc = spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+  if (!rc) {
+    const double h_v = 2.0 * L_v / (double)(N - 1), h_eta = (2.0 * M_PI / N) / h_v, L_eta = 0.5 * N * h_eta;
+    for (int i = 0; i < N; i++) {
+      eta[i] = -L_eta + (double)i * h_eta;
+      v[i] = -L_v + (double)i * h_v;
+    }
+    printf("Stored weights not found, generating...\n");
+    rc = spbte_op_create(&op, N, L_v, v, eta);
+  }
+  if (!rc) rc = spbte_op_generate_weights_landau(op, lambda);
+  if (!rc) rc = spbte_op_store_weights_file(op, name);
+  spbte_op_destroy(op);
+  free(v);
+  free(eta);
+  if (rc) {
+    fprintf(stderr, "spectralbte_amd: %s\n", spbte_last_error());
+    return 1;
+  }
+  printf("Wrote %s\n", name);
+  return 0;
 }
 
 static int boltz_run(const char *input_name, const char *flags_name) {

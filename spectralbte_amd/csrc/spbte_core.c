@@ -446,6 +446,22 @@ int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, doubl
   return SPBTE_OK;
 }
 
+int spbte_op_generate_weights_landau(spbte_op *op, double lambda) {
+  if (!op || !isfinite(lambda)) return fail(SPBTE_ERR_ARG, "bad arguments");
+  int rc = alloc_w64(op);
+  if (rc) return rc;
+  double *d_eta = NULL;
+  HIPCHK(hipMalloc((void **)&d_eta, sizeof(double) * op->N));
+  HIPCHK(hipMemcpy(d_eta, op->eta, sizeof(double) * op->N, hipMemcpyHostToDevice));
+  int e = spbte_launch_weights_landau(&op->g, d_eta, lambda, op->dW64, op->s);
+  hipError_t e2 = hipStreamSynchronize(op->s);
+  hipFree(d_eta);
+  if (e != 0 || e2 != hipSuccess)
+    return fail(SPBTE_ERR_HIP, "Landau weight generation: %s", hipGetErrorString(e ? (hipError_t)e : e2));
+  op->have_w = 1;
+  return SPBTE_OK;
+}
+
 int spbte_op_get_weights(spbte_op *op, double *W) {
   if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
   HIPCHK(hipStreamSynchronize(op->s));

@@ -172,3 +172,32 @@ def test_boltz_refuses_what_the_reference_cannot_do(spb, tmp_path):
         d = _workdir(tmp_path, f"bad{init}", {"run.in": text, "run.out": FLAGS, "m.mesh": "%m\n4\n1\n4\n1.0\n"})
         with pytest.raises(spb.SpbteError, match=msg):
             spb.boltz("run.in", "run.out", workdir=d)
+
+
+def test_boltz_landau_table_generated_and_used(spb, orc, tmp_path):
+    """Anisotropic 1 with an empty Weights/: the Landau table (glance 0, as
+    exec/boltz.c:151-153 asks for) is generated on the GPU, stored under the
+    reference's name, and the 0D run with it matches the oracle's run with the
+    stored table; weightgen_amd (exec/MPIWeightGenerator.c) writes the same
+    file bit for bit."""
+    N, Lv, dt, nT = 8, 5.0, 1e-6, 3
+    text = _input_text(N=N, L_v=Lv, Knudsen=1.0, Lambda=1.0, Time_step=dt, Number_of_time_steps=nT, Space_order=2,
+                       Data_writing_frequency=1, Init_field=2, SpaceInhom=0, Anisotropic=1)
+    d = _workdir(tmp_path, "landau", {"run.in": text, "run.out": os.path.join(GOLDEN, "BKW8", "BKW8.test.out")})
+    spb.boltz("run.in", "run.out", workdir=d)
+    name = "N8_Aniso_L_v5_lambda1_Landau.wts"
+    W = orc.weights_load(str(d / "Weights" / name), N)
+    v, eta = orc.grid_0d(N, Lv)
+    for r in (3, 200, 511):
+        ref = orc.weights_landau(N, eta, Lv, 1.0, rows=(r, 1))[0]
+        assert np.abs(W[r] - ref).max() <= 1e-12 * np.abs(ref).max()
+    ref_rows = np.array(orc.run_0d(N, v, eta, W, orc.bkw_initial(N, v).ravel(), dt, nT, order=2))
+    got = _rows(d / "Data" / "moments_run.in")
+    assert got.shape == ref_rows.shape
+    m = np.abs(ref_rows) > 1e-12
+    assert (np.abs(got - ref_rows)[m] / np.abs(ref_rows)[m]).max() < 1e-6
+    g = tmp_path / "gen"
+    (g / "Weights").mkdir(parents=True)
+    spb.weightgen(N, 0, 1, 1.0, Lv, workdir=g)
+    assert np.array_equal(np.fromfile(g / "Weights" / name), np.fromfile(d / "Weights" / name))
+    spb.weightgen(N, 0, 1, 1.0, Lv, workdir=g)  # an existing table is kept

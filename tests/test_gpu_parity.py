@@ -326,3 +326,26 @@ def test_n32_single_cell_against_oracle(spb, orc):
         err = rel_l2(op.compute_q(f[None], form=form)[0], q64)
         print(form, err)
         assert err < 1e-6, (form, err)
+
+
+@pytest.mark.parametrize("N,Lv,lam,rows", [(8, 5.0, 1.0, list(range(0, 512, 37))), (12, 4.0, 0.0, [0, 871, 1727])])
+def test_weights_landau_matches_oracle(spb, orc, N, Lv, lam, rows):
+    """The glance = 0 (Landau) table generated on the GPU against the oracle's
+    gHat3L rows (1e-12 of the row's max |W|: the same 64-point rules, GPU
+    Bessel J0 against the C library's), and its xi = 0 column against the
+    closed form -(8 pi / 3) |zeta|^2 L^(lam+5) / (lam+5) for every row."""
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_landau(lam)
+    W = op.get_weights()
+    for r in rows:
+        ref = orc.weights_landau(N, eta, Lv, lam, rows=(r, 1))[0]
+        assert np.abs(W[r] - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-300), r
+    h = N // 2
+    c0 = h + N * (h + N * h)
+    assert eta[h] == 0.0
+    I, J, K = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    z2 = (eta[I] ** 2 + eta[J] ** 2 + eta[K] ** 2).ravel()
+    want = -(8 * np.pi / 3) * z2 * Lv ** (lam + 5) / (lam + 5)
+    assert np.abs(W[:, c0] - want).max() <= 1e-13 * np.abs(want).max()
+    op.close()

@@ -274,3 +274,44 @@ def test_poiseuille_force_adds_y_momentum(orc):
         d = forced[l] - base[l]
         assert abs(W3 @ (VY * d) - Ma * 0.5 * dt * rho) < 1e-3 * Ma * 0.5 * dt * rho
         assert abs(W3 @ d) < 1e-9 and abs(W3 @ (VX * d)) < 1e-9 and abs(W3 @ (VZ * d)) < 1e-9
+
+
+def test_gauss_legendre64_rule(orc):
+    """The 64-point rule of gsl_integration_glfixed / gauss_legendre: published
+    nodes and weights (Abramowitz & Stegun 25.4 to double precision), weights
+    summing to 2, and exactness on even powers up to degree 126."""
+    x, w = orc.gauss_legendre64()
+    assert x[0] == 0.0243502926634244325089558 and w[0] == 0.0486909570091397203833654
+    assert x[31] == 0.9993050417357721394569056 and w[31] == 0.0017832807216964329472961
+    assert abs(2 * w.sum() - 2) < 1e-15
+    for k in (2, 10, 64, 126):
+        assert abs(2 * (w * x ** k).sum() - 2 / (k + 1)) < 5e-15 * 2 / (k + 1), k
+
+
+@pytest.mark.parametrize("lam", [0.0, 1.0, 2.0])
+def test_landau_entry_closed_form_and_quadrature(orc, lam):
+    """gHat3L (exec/MPIcollisionRoutines.c:205-248): with the column mode xi = 0
+    the integrand is -r^(lam+4) |zeta|^2 sin^3(theta), so the entry is
+    -(8 pi / 3) |zeta|^2 L^(lam+5) / (lam+5); a zero row mode gives 0; generic
+    entries agree with scipy's adaptive 2D quadrature of the same integral."""
+    import math
+    from scipy import integrate, special
+    Lv = 5.0
+    z = (0.3, -0.7, 1.1)
+    z2 = sum(t * t for t in z)
+    want = -(8 * math.pi / 3) * z2 * Lv ** (lam + 5) / (lam + 5)
+    assert abs(orc.landau_entry(z, (0, 0, 0), Lv, lam) / want - 1) < 1e-13
+    assert orc.landau_entry((0, 0, 0), (0.55, 1.1, -0.55), Lv, lam) == 0.0
+    for zeta, xi in [((0.55, -1.1, 0.55), (1.65, 0.55, -0.55)), ((-1.65, 0.0, 0.55), (0.55, 2.2, 1.1))]:
+        zeta, xi = np.array(zeta), np.array(xi)
+        zl = np.linalg.norm(zeta)
+        a2 = xi @ zeta / zl
+        xp = math.sqrt(max(xi @ xi - a2 * a2, 0.0))
+
+        def f(th, r):
+            s, c = math.sin(th), math.cos(th)
+            return r ** (lam + 2) * s * special.j0(r * xp * s) * (
+                -r * r * zl * zl * s * s * math.cos(r * a2 * c) + 4 * r * zl * math.sin(r * a2 * c) * c)
+        ref = 2 * math.pi * integrate.dblquad(f, 0, Lv, 0, math.pi, epsabs=1e-12, epsrel=1e-12)[0]
+        got = orc.landau_entry(zeta, xi, Lv, lam)
+        assert abs(got - ref) < 1e-10 * abs(ref), (got, ref)
