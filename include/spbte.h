@@ -81,6 +81,12 @@ int spbte_op_set_species(spbte_op *op, int Ns, const double *masses);
  * isotropic table on the GPU (src/weights.c:265-281 closed forms for lambda 0
  * and 1, quadrature otherwise) for species diameters d_i, d_j and masses m_i, m_j.
  * get_weights copies the FP64 table back to host memory (contiguous). */
+/* Table slots: an operator holds up to 64 tables on its grid (8 species),
+ * one per species pair of a multi-species run (slot m*Ns + l for the pair
+ * (l, m), the reference's conv_weights[m*Ns + l], exec/boltz.c:157-160).
+ * The table calls below and spbte_compute_q act on the selected slot
+ * (slot 0 until another is selected). */
+int spbte_op_select_table(spbte_op *op, int slot);
 int spbte_op_set_weights(spbte_op *op, const double *W);
 int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows);
 int spbte_op_load_weights_file(spbte_op *op, const char *path);
@@ -105,6 +111,11 @@ int spbte_conserve(spbte_op *op, double *d_Q, int ncell);
 
 /* General projection of one group of Ns*Ns arrays (array n*Ns+m), in place. */
 int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups);
+/* The same projection for a pair-major batch: pair (l, m) of cell c at
+ * d_Q + ((m*Ns + l) * ncell + c) * N^3, the layout a multi-species run keeps. */
+int spbte_conserve_pairs(spbte_op *op, double *d_Q, int ncell);
+/* Species count and masses set by spbte_op_set_species. */
+int spbte_op_get_species(spbte_op *op, int *Ns, double *masses);
 
 /* The transform of src/collisions.c:232-283 on ncell complex cubes
  * (interleaved re/im doubles); invert 0 forward, 1 inverse.  d_in may equal d_out. */
@@ -159,18 +170,31 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
  * table, species or tiles change.  The run must not outlive its operator. */
 enum { SPBTE_BC_COPY = 0, SPBTE_BC_DIFFUSE = 1, SPBTE_BC_PERIODIC = 2, SPBTE_BC_DIFFUSE_LEFT = 3 };
 
+/* Several species (exec/boltz.c:193-241, :254-354 with Ns > 1): the operator
+ * holds Ns species (spbte_op_set_species) and the table of pair (l, m) in slot
+ * m*Ns + l; a step evaluates Q(f_l, f_m) for every pair, projects them jointly
+ * (Ns + 4 moments, src/conserve.c:207-264) and updates f_l with the sum over m
+ * of Q(l, m).  Walls reflect each species with its own mass.  The state of the
+ * run (set_state, get_state, moments) is species-major: species l's cells
+ * follow species l-1's. */
+
 typedef struct spbte_run spbte_run;
 
 int spbte_run0d_create(spbte_run **run, spbte_op *op, int ncell, int order, double dt, double Kn, int form,
                        int maxpreserve);
 int spbte_run1d_create(spbte_run **run, spbte_op *op, int nX, int order, const double *x, const double *dx, double dt,
                        double Kn, int bc, double T0, double T1, int form);
+int spbte_run0d_create_species(spbte_run **run, spbte_op *op, int Ns, int ncell, int order, double dt, double Kn,
+                               int form, int maxpreserve);
+int spbte_run1d_create_species(spbte_run **run, spbte_op *op, int Ns, int nX, int order, const double *x,
+                               const double *dx, double dt, double Kn, int bc, double T0, double T1, int form);
 void spbte_run_destroy(spbte_run *run);
-/* Interior cells, ncell * N^3 host doubles (synchronous). */
+/* Interior cells, Ns * ncell * N^3 host doubles (synchronous). */
 int spbte_run_set_state(spbte_run *run, const double *f);
 int spbte_run_get_state(spbte_run *run, double *f);
-/* Device pointer to the interior cells of the state. */
+/* Device pointer to the interior cells of the state (species 0, or species l). */
 int spbte_run_state(spbte_run *run, double **d_f);
+int spbte_run_species_state(spbte_run *run, int l, double **d_f);
 int spbte_run_use_graph(spbte_run *run, int on);
 /* Queue nsteps steps (asynchronous; spbte_op_synchronize waits). */
 int spbte_run_step(spbte_run *run, int nsteps);
@@ -179,7 +203,8 @@ int spbte_run_step(spbte_run *run, int nsteps);
  * Ma = 0 turns it off.  Order-1 runs are refused: the reference applies that
  * term after its k loop, outside the cell (:218-226). */
 int spbte_run_set_forcing(spbte_run *run, double Ma);
-/* Per cell rho, u_x, u_y, u_z, T (src/momentRoutines.c:58-183): 5 * ncell host doubles. */
+/* Per species and cell rho, u_x, u_y, u_z, T with unit mass
+ * (src/momentRoutines.c:58-183): 5 * Ns * ncell host doubles. */
 int spbte_run_moments(spbte_run *run, double *out);
 
 /* Moment output in the reference's format (src/output.c:64-213, :215-404):
@@ -201,14 +226,21 @@ enum {
 };
 int spbte_read_output_flags(const char *path, int *flags);
 int spbte_run_open_output(spbte_run *run, const char *path, int flags, int append);
+/* The file of species l of a multi-species run (src/output.c:138-157 names
+ * them moments_<input>_<species name>); write_output writes every open file
+ * of the run and close_output closes them all. */
+int spbte_run_open_output_species(spbte_run *run, int l, const char *path, int flags, int append);
 int spbte_run_write_output(spbte_run *run, double time);
 int spbte_run_close_output(spbte_run *run);
 
-/* Restart dumps (src/restart.c:24-109): the interior cells' N^3 doubles back
- * to back in pdf_path, and the step counter as one int in time_path (either
- * may be skipped with a null path on load). */
+/* Restart dumps (src/restart.c:24-109): species 0's interior cells' N^3
+ * doubles back to back in pdf_path, and the step counter as one int in
+ * time_path (time_path may be null). */
 int spbte_run_store_restart(spbte_run *run, const char *pdf_path, const char *time_path, int step);
 int spbte_run_load_restart(spbte_run *run, const char *pdf_path, const char *time_path, int *step);
+/* The same for species l's cells only (one file per species, src/restart.c:41-54). */
+int spbte_run_store_restart_species(spbte_run *run, int l, const char *pdf_path);
+int spbte_run_load_restart_species(spbte_run *run, int l, const char *pdf_path);
 
 /* One transport stage alone, synchronous: order 1 = upwindOne, order 2 =
  * advectTwo, from d_f to d_fc, both (nX + 2*order) * N^3 device doubles with

@@ -60,6 +60,7 @@ def lib():
             "orc_ghat_integral": (D, [D, D, D, D, D]),
             "orc_weights_iso": (None, [I, _dp, D, D, D, D, D, D, _dp, I]),
             "orc_gauss_legendre64": (None, [_dp, _dp]),
+            "orc_set_wall_kb": (None, [D]),
             "orc_landau_entry": (D, [D, D, D, D, D, D, D, D]),
             "orc_weights_landau_rows": (None, [I, _dp, D, D, ctypes.c_long, ctypes.c_long, _dp]),
             "orc_weights_load": (I, [ctypes.c_char_p, I, _dp]),
@@ -299,6 +300,107 @@ def run_0d(N, v, eta, W, f0, dt, nsteps, order=2, Kn=1.0, out_every=1, nthreads=
         if (t + 1) % out_every == 0:
             rows.append(moments_row_0d(N, v, f, dt * (t + 1)))
     return rows
+
+
+def run_0d_species(N, v, eta, Ws, f0, masses, dt, nsteps, order=2, Kn=1.0, nthreads=None):
+    """exec/boltz.c:193-241 with Ns species: Ws[m*Ns + l] is the table of pair
+    (l, m), Q[m*Ns + l] = ComputeQ_maxPreserve(f_l, f_m) (whose Maxwellian uses
+    species 0's mass, src/collisions.c:95-97), the joint projection, and f_l
+    updated with Q[m*Ns + l] in m order.  f0: (Ns, N^3).  Returns the states
+    (Ns, N^3) after each step."""
+    Ns = len(masses)
+    cons = Conserver(N, v, masses)
+    f = [np.array(x, dtype=np.float64).reshape(-1).copy() for x in f0]
+
+    def all_q(ff):
+        Q = np.zeros((Ns * Ns, N ** 3))
+        for l in range(Ns):
+            for m in range(Ns):
+                Q[m * Ns + l] = compute_q_maxpreserve(N, v, eta, Ws[m * Ns + l], ff[l], ff[m], mass=masses[0],
+                                                      nthreads=nthreads).ravel()
+        return cons(Q)
+
+    out = []
+    for _ in range(nsteps):
+        Q = all_q(f)
+        if order == 1:
+            for l in range(Ns):
+                for m in range(Ns):
+                    f[l] = f[l] + dt * Q[m * Ns + l] / Kn
+        else:
+            f1 = [x.copy() for x in f]
+            for l in range(Ns):
+                for m in range(Ns):
+                    f1[l] = f1[l] + dt * Q[m * Ns + l] / Kn
+            Q = all_q(f1)
+            for l in range(Ns):
+                f[l] = 0.5 * (f[l] + f1[l])
+                for m in range(Ns):
+                    f[l] = f[l] + 0.5 * dt * Q[m * Ns + l] / Kn
+        out.append(np.array(f))
+    return out
+
+
+def run_1d_species(N, v, eta, Ws, f0, masses, x, dx, dt, nsteps, order, Kn, bc, T0=1.0, T1=2.0, nthreads=None):
+    """exec/boltz.c:254-354 with Ns species on one rank: each species
+    transported with its own mass at the walls (src/boundaryConditions.c:56-62),
+    per cell Q[n*Ns + m] = ComputeQ(f_m, f_n) with table Ws[n*Ns + m], the joint
+    projection, Euler or Heun.  f0: (Ns, nX, N^3).  Returns the interior states
+    (Ns, nX, N^3) after each step."""
+    Ns = len(masses)
+    f0 = _c(f0)
+    nX = f0.shape[1]
+    n3 = N ** 3
+    cons = Conserver(N, v, masses)
+    f = []
+    for m in range(Ns):
+        a = np.zeros((nX + 2 * order, n3))
+        a[order:order + nX] = f0[m]
+        f.append(a)
+
+    def cell_q(cells):
+        Q = np.zeros((Ns * Ns, n3))
+        for m in range(Ns):
+            for n in range(Ns):
+                Q[n * Ns + m] = compute_q(N, v, eta, Ws[n * Ns + m], cells[m], cells[n], nthreads=nthreads).ravel()
+        return cons(Q)
+
+    lib().orc_set_wall_kb(1.0 if masses[0] == 1.0 else 1.380658e-23)
+    out = []
+    for _ in range(nsteps):
+        fc = []
+        for m in range(Ns):
+            if order == 1:
+                f[m], c = upwind_one(N, nX, v, dx, dt, bc, f[m], T0, T1, mass=masses[m])
+            else:
+                f[m], c = advect_two(N, nX, v, x, dx, dt, bc, f[m], T0, T1, mass=masses[m])
+            fc.append(c)
+        for l in range(order, nX + order):
+            Q = cell_q([fc[m][l] for m in range(Ns)])
+            if order == 1:
+                for m in range(Ns):
+                    f[m][l] = fc[m][l]
+                    for n in range(Ns):
+                        f[m][l] = f[m][l] + dt * Q[n * Ns + m] / Kn
+            else:
+                f1 = []
+                for m in range(Ns):
+                    a = fc[m][l].copy()
+                    for n in range(Ns):
+                        a = a + dt * Q[n * Ns + m] / Kn
+                    f1.append(a)
+                Q = cell_q(f1)
+                for m in range(Ns):
+                    fc[m][l] = 0.5 * fc[m][l] + 0.5 * f1[m]
+                    for n in range(Ns):
+                        fc[m][l] = fc[m][l] + 0.5 * dt * Q[n * Ns + m] / Kn
+        if order == 2:
+            for m in range(Ns):
+                _, fnew = advect_two(N, nX, v, x, dx, dt, bc, fc[m].copy(), T0, T1, mass=masses[m])
+                f[m][order:order + nX] = fnew[order:order + nX]
+        out.append(np.array([f[m][order:order + nX] for m in range(Ns)]))
+    lib().orc_set_wall_kb(0.0)
+    return out
 
 
 def uniform_mesh(nX, length, order):

@@ -655,8 +655,14 @@ static double minmod3(double a, double b, double c) {
  * of `in` through the wall sets sigma_W; `out` gets sigma_W times a wall
  * Maxwellian exp(-m|v|^2/(2 KB T_W)) on the incoming half (left wall: i >= N/2;
  * right wall: i < N/2).  The other half of `out` is left untouched. */
+/* KB of the walls: src/boundaryConditions.c:31-34 takes it from species 0's
+ * mass (1 for mass 1, SI otherwise) for every species; orc_set_wall_kb sets it
+ * for the species-mass walls of a multi-species run (0: from `mass`). */
+static double orc_wall_kb = 0.0;
+void orc_set_wall_kb(double kb) { orc_wall_kb = kb; }
+
 void orc_diffuse_bc(int N, const double *v, double mass, const double *in, double *out, double TW, int right) {
-  double w[64], hv = v[1] - v[0], KB = kb_for_mass(mass), sig = 0.0;
+  double w[64], hv = v[1] - v[0], KB = orc_wall_kb > 0 ? orc_wall_kb : kb_for_mass(mass), sig = 0.0;
   trap_weights(N, w);
   int o0 = right ? N / 2 : 0, o1 = right ? N : N / 2;
   for (int i = o0; i < o1; i++)

@@ -143,6 +143,15 @@ def lib():
             "spbte_run_set_forcing": (I, [_vp, D]),
             "spbte_op_store_weights_file": (I, [_vp, ctypes.c_char_p]),
             "spbte_op_generate_weights_landau": (I, [_vp, D]),
+            "spbte_op_select_table": (I, [_vp, I]),
+            "spbte_conserve_pairs": (I, [_vp, _vp, I]),
+            "spbte_op_get_species": (I, [_vp, ip, _dp]),
+            "spbte_run0d_create_species": (I, [ctypes.POINTER(_vp), _vp, I, I, I, D, D, I, I]),
+            "spbte_run1d_create_species": (I, [ctypes.POINTER(_vp), _vp, I, I, I, _dp, _dp, D, D, I, D, D, I]),
+            "spbte_run_species_state": (I, [_vp, I, ctypes.POINTER(_vp)]),
+            "spbte_run_open_output_species": (I, [_vp, I, ctypes.c_char_p, I, I]),
+            "spbte_run_store_restart_species": (I, [_vp, I, ctypes.c_char_p]),
+            "spbte_run_load_restart_species": (I, [_vp, I, ctypes.c_char_p]),
             "spbte_weightgen_main": (I, [I, ctypes.POINTER(ctypes.c_char_p)]),
             # front end (spbte_boltz.h)
             "spbte_read_input": (I, [ctypes.c_char_p, _vp]),
@@ -248,6 +257,11 @@ class Operator:
     def set_species(self, masses):
         m = _c(masses)
         _check(lib().spbte_op_set_species(self.h, len(m), _p(m)))
+
+    def select_table(self, slot):
+        """Table slot for the table calls and evaluations (pair (l, m) of Ns
+        species in slot m*Ns + l)."""
+        _check(lib().spbte_op_select_table(self.h, int(slot)))
 
     def set_weights(self, W):
         W = _c(W)
@@ -372,8 +386,8 @@ class Run:
     (exec/boltz.c:193-241 for 0D, :285-354 for 1D).  Create with ``Run.ensemble``
     or ``Run.mesh``."""
 
-    def __init__(self, op, handle, ncell):
-        self.op, self.h, self.ncell = op, handle, ncell
+    def __init__(self, op, handle, ncell, ns=1):
+        self.op, self.h, self.ncell, self.ns = op, handle, ncell, ns
         self.n3 = op.N ** 3
 
     @classmethod
@@ -399,15 +413,43 @@ class Run:
         r.set_state(f0)
         return r
 
+    @classmethod
+    def ensemble_species(cls, op, f0, order=2, dt=0.01, Kn=1.0, form="fp64", maxpreserve=True):
+        """Ns species (op.set_species, pair tables in slots m*Ns + l): f0 is
+        (Ns, ncell, N^3)."""
+        f0 = _c(f0)
+        ns, ncell = f0.shape[0], f0.shape[1]
+        h = _vp()
+        _check(lib().spbte_run0d_create_species(ctypes.byref(h), op.h, ns, ncell, order, dt, Kn, FORMS[form],
+                                                int(maxpreserve)))
+        r = cls(op, h, ncell, ns)
+        r.set_state(f0)
+        return r
+
+    @classmethod
+    def mesh_species(cls, op, f0, x, dx, order=1, dt=0.01, Kn=1.0, bc=BC_DIFFUSE, T0=1.0, T1=2.0, form="fp64"):
+        """Ns species on a 1D mesh: f0 is (Ns, nX, N^3)."""
+        f0 = _c(f0)
+        ns, nX = f0.shape[0], f0.shape[1]
+        x, dx = _c(x), _c(dx)
+        assert x.size == nX + 2 * order and dx.size == x.size
+        h = _vp()
+        _check(lib().spbte_run1d_create_species(ctypes.byref(h), op.h, ns, nX, order, _p(x), _p(dx), dt, Kn, bc, T0,
+                                                T1, FORMS[form]))
+        r = cls(op, h, nX, ns)
+        r.set_state(f0)
+        return r
+
     def set_state(self, f):
         f = _c(f)
-        assert f.size == self.ncell * self.n3
+        assert f.size == self.ns * self.ncell * self.n3
         _check(lib().spbte_run_set_state(self.h, _p(f)))
 
     def state(self):
-        out = np.empty((self.ncell, self.n3))
+        """(ncell, N^3), or (Ns, ncell, N^3) for several species."""
+        out = np.empty((self.ns, self.ncell, self.n3))
         _check(lib().spbte_run_get_state(self.h, _p(out)))
-        return out
+        return out[0] if self.ns == 1 else out
 
     def state_ptr(self):
         p = _vp()
@@ -457,10 +499,19 @@ class Run:
         return step.value
 
     def moments(self):
-        """(ncell, 5): rho, u_x, u_y, u_z, T per cell."""
-        out = np.empty((self.ncell, 5))
+        """(ncell, 5): rho, u_x, u_y, u_z, T per cell (unit mass); (Ns, ncell, 5) for several species."""
+        out = np.empty((self.ns, self.ncell, 5))
         _check(lib().spbte_run_moments(self.h, _p(out)))
-        return out
+        return out[0] if self.ns == 1 else out
+
+    def open_output_species(self, l, path, flags=OUT_ALL, append=False):
+        _check(lib().spbte_run_open_output_species(self.h, int(l), os.fsencode(path), int(flags), int(append)))
+
+    def store_restart_species(self, l, pdf_path):
+        _check(lib().spbte_run_store_restart_species(self.h, int(l), os.fsencode(pdf_path)))
+
+    def load_restart_species(self, l, pdf_path):
+        _check(lib().spbte_run_load_restart_species(self.h, int(l), os.fsencode(pdf_path)))
 
     def close(self):
         if self.h:

@@ -208,14 +208,15 @@ struct ProjArgs {
 };
 
 __global__ void __launch_bounds__(XT) k_project(int N, int Ns, const double *__restrict__ v, const double *__restrict__ w,
-                                                double hv, ProjArgs pa, double *__restrict__ Q) {
+                                                double hv, ProjArgs pa, double *__restrict__ Q, size_t gstride,
+                                                size_t pstride) {
   __shared__ double red[(SPBTE_MAX_NS + 4) * (XT / 64)];
   __shared__ double lam[SPBTE_MAX_NS + 4];
   const int grp = blockIdx.x;
   const int n2 = N * N;
   const size_t n3 = (size_t)n2 * N;
   const int nel = Ns + 4;
-  double *Qg = Q + (size_t)grp * Ns * Ns * n3;
+  double *Qg = Q + (size_t)grp * gstride;  // pair p of this group at Qg + p * pstride
   const double dv3 = hv * hv * hv;
   double s[SPBTE_MAX_NS + 4];
   for (int q = 0; q < SPBTE_MAX_NS + 4; ++q) s[q] = 0.0;
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(XT) k_project(int N, int Ns, const double *__r
     const double e2 = 0.5 * (v[i] * v[i] + v[j] * v[j] + v[k] * v[k]);
     for (int m = 0; m < Ns; ++m) {
       double qs = 0.0;
-      for (int n = 0; n < Ns; ++n) qs += Qg[(size_t)(n * Ns + m) * n3 + e];
+      for (int n = 0; n < Ns; ++n) qs += Qg[(size_t)(n * Ns + m) * pstride + e];
       const double p = wv * pa.masses[m] * qs;
       s[m] += p;
       s[Ns] += p * v[i];
@@ -260,7 +261,7 @@ __global__ void __launch_bounds__(XT) k_project(int N, int Ns, const double *__r
     for (int m = 0; m < Ns; ++m) {
       const double p = wv * pa.masses[m] / Ns;
       const double corr = p * lam[m] + p * v[i] * lam[Ns] + p * v[j] * lam[Ns + 1] + p * v[k] * lam[Ns + 2] + p * e2 * lam[Ns + 3];
-      for (int n = 0; n < Ns; ++n) Qg[(size_t)(n * Ns + m) * n3 + e] -= corr;
+      for (int n = 0; n < Ns; ++n) Qg[(size_t)(n * Ns + m) * pstride + e] -= corr;
     }
   }
 }
@@ -318,13 +319,24 @@ extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *
 
 extern "C" int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                                     double *Q, int ncell, hipStream_t s) {
+  const size_t n3 = (size_t)g->N * g->N * g->N;
+  return spbte_launch_project_strided(g, Ns, masses, LU, pivot, Q, ncell, (size_t)Ns * Ns * n3, n3, s);
+}
+
+// Groups at Q + grp * gstride, the Ns^2 pairs of a group pstride apart
+// (group-major: gstride = Ns^2 N^3, pstride = N^3; pair-major over ncell
+// cells: gstride = N^3, pstride = ncell N^3).
+extern "C" int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU,
+                                            const int *pivot, double *Q, int ngroups, size_t gstride, size_t pstride,
+                                            hipStream_t s) {
   if (Ns < 1 || Ns > SPBTE_MAX_NS) return (int)hipErrorInvalidValue;
+  const int ncell = ngroups;
   ProjArgs pa = {};
   const int nel = Ns + 4;
   for (int q = 0; q < nel * nel; ++q) pa.LU[q] = LU[q];
   for (int q = 0; q < nel; ++q) pa.pivot[q] = pivot[q];
   for (int q = 0; q < Ns; ++q) pa.masses[q] = masses[q];
   hipLaunchKernelGGL(k_project, dim3(ncell), dim3(XT), 0, s, g->N, Ns, (const double *)g->d_v, (const double *)g->d_w, g->hv,
-                     pa, Q);
+                     pa, Q, gstride, pstride);
   return (int)hipGetLastError();
 }

@@ -373,15 +373,15 @@ static int walls_of(int init, int order, double *T0, double *T1) {
  * generated on the GPU and stored there: isotropic (src/weights.c:65-124) or,
  * for Anisotropic 1, the glance = 0 Landau table (src/aniso_weights.c:37-146,
  * which boltz_ calls with glance 0). */
-static int setup_weights(spbte_op *op, const spbte_input *in, const species *sp) {
+static int setup_weights(spbte_op *op, const spbte_input *in, const species *si, const species *sj) {
   char name[512];
   if (in->aniso)
     snprintf(name, sizeof name, "Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts", in->N, in->L_v, in->lambda);
-  else if (!strcmp(sp->name, "default"))
+  else if (!strcmp(si->name, "default"))
     snprintf(name, sizeof name, "Weights/N%d_isotropic_L_v%g_lambda%g.wts", in->N, in->L_v, in->lambda);
   else
-    snprintf(name, sizeof name, "Weights/N%d_isotropic_L_v%g_HS_%s_%zu_%s_%zu.wts", in->N, in->L_v, sp->name, sp->id,
-             sp->name, sp->id);
+    snprintf(name, sizeof name, "Weights/N%d_isotropic_L_v%g_HS_%s_%zu_%s_%zu.wts", in->N, in->L_v, si->name, si->id,
+             sj->name, sj->id);
   if (in->weight_flag == 0) {
     FILE *fp = fopen(name, "rb");
     if (fp) {
@@ -394,7 +394,7 @@ static int setup_weights(spbte_op *op, const spbte_input *in, const species *sp)
     printf("Fresh version of weights being computed and stored for this configuration\n");
   }
   if (in->aniso) CHK(spbte_op_generate_weights_landau(op, in->lambda));
-  else CHK(spbte_op_generate_weights_iso(op, in->lambda, sp->d_ref, sp->d_ref, sp->mass, sp->mass));
+  else CHK(spbte_op_generate_weights_iso(op, in->lambda, si->d_ref, sj->d_ref, si->mass, sj->mass));
   return spbte_op_store_weights_file(op, name);
 }
 
@@ -453,13 +453,17 @@ int spbte_weightgen_main(int argc, char **argv) {
 }
 
 static int boltz_run(const char *input_name, const char *flags_name) {
-  char path[1024], pdf[1024], tpath[1024];
+  char path[1024], tpath[1024];
   spbte_input in;
   snprintf(path, sizeof path, "./input/%s", input_name);
   printf("Opening input file %s\n", path);
   CHK(spbte_read_input(path, &in));
-  if (in.num_species != 1)
-    return spbte_fail(SPBTE_ERR_UNSUPPORTED, "num_species %d: runs of one species only", in.num_species);
+  const int Ns = in.num_species;
+  if (Ns > 1 && in.aniso)
+    return spbte_fail(SPBTE_ERR_UNSUPPORTED,
+                      "Anisotropic 1 with %d species: the reference loads one table for the first pair only "
+                      "(exec/boltz.c:151-153)",
+                      Ns);
   if (in.N < 2 || !(in.L_v > 0) || !(in.Kn > 0) || !(in.dt > 0) || in.nT < 0 || in.data_freq < 1 ||
       (in.order != 1 && in.order != 2) || in.inhom < 0 || in.inhom > 1)
     return spbte_fail(SPBTE_ERR_ARG,
@@ -470,12 +474,17 @@ static int boltz_run(const char *input_name, const char *flags_name) {
   const int form = form_from_env();
   if (form < 0) return spbte_fail(SPBTE_ERR_ARG, "SPBTE_FORM must be fp64, fp32, fp16x2 or fp32a");
 
-  species sp;
-  spbte_species_defaults(&sp, 0);
-  if (strcmp(in.species_names[0], "default") != 0) {
-    snprintf(path, sizeof path, "./input/species/%s", in.species_names[0]);
-    printf("Opening species input file %s\n", path);
-    CHK(spbte_read_species(path, &sp));
+  /* load_and_allocate_spec, src/species.c:45-160 */
+  species sp[SPBTE_MAX_SPECIES];
+  double masses[SPBTE_MAX_SPECIES];
+  for (int l = 0; l < Ns; l++) {
+    spbte_species_defaults(&sp[l], l);
+    if (strcmp(in.species_names[l], "default") != 0) {
+      snprintf(path, sizeof path, "./input/species/%s", in.species_names[l]);
+      printf("Opening species input file %s\n", path);
+      CHK(spbte_read_species(path, &sp[l]));
+    }
+    masses[l] = sp[l].mass;
   }
 
   const int N = in.N;
@@ -499,49 +508,65 @@ static int boltz_run(const char *input_name, const char *flags_name) {
   }
   if (!rc) {
     spbte_velocity_grid(N, in.L_v, in.inhom, v, eta);
-    f = malloc(sizeof(double) * n3 * nX);
-    rc = f ? spbte_initial_state(&in, &sp, v, nX, f) : spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+    f = malloc(sizeof(double) * n3 * nX * Ns);
+    if (!f) rc = spbte_fail(SPBTE_ERR_ARG, "out of host memory");
+    for (int l = 0; l < Ns && !rc; l++) {
+      /* the states' Boltzmann constant follows species 0's name (src/initializer.c:52-55) */
+      species sl = sp[l];
+      strcpy(sl.name, sp[0].name);
+      rc = spbte_initial_state(&in, &sl, v, nX, f + (size_t)l * nX * n3);
+    }
   }
   if (!rc) rc = spbte_op_create(&op, N, in.L_v, v, eta);
-  if (!rc) rc = spbte_op_set_species(op, 1, &sp.mass);
-  if (!rc) rc = setup_weights(op, &in, &sp);
+  if (!rc) rc = spbte_op_set_species(op, Ns, masses);
+  /* the table of pair (i, j) in slot j*Ns + i, exec/boltz.c:157-160 */
+  for (int i = 0; i < Ns && !rc; i++)
+    for (int j = 0; j < Ns && !rc; j++) {
+      rc = spbte_op_select_table(op, j * Ns + i);
+      if (!rc) rc = setup_weights(op, &in, &sp[i], &sp[j]);
+    }
+  if (!rc) rc = spbte_op_select_table(op, 0);
 
   double T0 = 1.0, T1 = 2.0;
   if (!rc && !in.inhom) {
-    rc = spbte_run0d_create(&run, op, 1, in.order, in.dt, in.Kn, form, 1);
+    rc = spbte_run0d_create_species(&run, op, Ns, 1, in.order, in.dt, in.Kn, form, 1);
   } else if (!rc) {
     const int bc = walls_of(in.init_flag, in.order, &T0, &T1);
     if (in.init_flag == 5 && in.order == 1)
       rc = spbte_fail(SPBTE_ERR_UNSUPPORTED,
                       "Init_field 5 (Poiseuille) needs Space_order 2: the reference's order-1 body force is applied "
                       "outside the cell (src/transportroutines.c:218-226)");
-    if (!rc) rc = spbte_run1d_create(&run, op, nX, in.order, x, dx, in.dt, in.Kn, bc, T0, T1, form);
+    if (!rc) rc = spbte_run1d_create_species(&run, op, Ns, nX, in.order, x, dx, in.dt, in.Kn, bc, T0, T1, form);
     if (!rc && in.init_flag == 5) rc = spbte_run_set_forcing(run, 1.0); /* Ma = 1, :176-179 */
   }
   if (!rc) rc = spbte_run_set_state(run, f);
 
-  /* restart (1D, src/initializer.c:287-296 and src/restart.c:65-109) */
+  /* restart (1D, src/initializer.c:287-296 and src/restart.c): one pdf per species */
   int t = 0;
-  snprintf(pdf, sizeof pdf, "Restart/%s_rank0_%s.plt", input_name, sp.name);
+  char pdf[SPBTE_MAX_SPECIES][1024];
+  for (int l = 0; l < Ns; l++) snprintf(pdf[l], sizeof pdf[l], "Restart/%s_rank0_%s.plt", input_name, sp[l].name);
   snprintf(tpath, sizeof tpath, "Restart/%s_time.plt", input_name);
   if (!rc && in.inhom && in.restart == 1) {
-    printf("Loading from previously generated data %s\n", pdf);
-    rc = spbte_run_load_restart(run, pdf, tpath, &t);
+    printf("Loading from previously generated data %s\n", pdf[0]);
+    rc = spbte_run_load_restart(run, pdf[0], tpath, &t);
+    for (int l = 1; l < Ns && !rc; l++) rc = spbte_run_load_restart_species(run, l, pdf[l]);
     if (!rc && (t < 0 || t > in.nT)) rc = spbte_fail(SPBTE_ERR_IO, "%s holds step %d of %d", tpath, t, in.nT);
   }
   if (!rc) rc = spbte_run_use_graph(run, 1);
 
-  /* output (src/output.c:64-137, :215-293) */
+  /* output (src/output.c:64-157, :215-293): one file per species, named
+   * moments_<input> for one species and moments_<input>_<name> for several */
   int flags = 0, opened = 0;
   if (!rc) {
     snprintf(path, sizeof path, "./input/%s", flags_name);
     rc = spbte_read_output_flags(path, &flags);
   }
-  if (!rc) {
-    snprintf(path, sizeof path, "Data/moments_%s", input_name);
+  for (int l = 0; l < Ns && !rc; l++) {
+    if (Ns > 1) snprintf(path, sizeof path, "Data/moments_%s_%s", input_name, sp[l].name);
+    else snprintf(path, sizeof path, "Data/moments_%s", input_name);
     printf("Opening output file %s\n", path);
-    rc = spbte_run_open_output(run, path, flags, in.restart != 0);
-    opened = !rc;
+    rc = spbte_run_open_output_species(run, l, path, flags, in.restart != 0);
+    opened |= !rc;
   }
   if (!rc && (!in.inhom || !in.restart)) rc = spbte_run_write_output(run, 0.0);
 
@@ -565,7 +590,8 @@ static int boltz_run(const char *input_name, const char *flags_name) {
       if (!rc && tot + write_time > 0.95 * in.restart_time) {
         printf("RESTART TIME REACHED - STORING CURRENT DISTRIBUTION DATA\n");
         /* the reference stores the index of the step just taken, t - 1 here */
-        rc = spbte_run_store_restart(run, pdf, tpath, t - 1);
+        rc = spbte_run_store_restart(run, pdf[0], tpath, t - 1);
+        for (int l = 1; l < Ns && !rc; l++) rc = spbte_run_store_restart_species(run, l, pdf[l]);
         if (!rc) printf("Time stored: %d\n", t - 1);
         break;
       }

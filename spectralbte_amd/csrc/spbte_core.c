@@ -62,11 +62,14 @@ struct spbte_op {
   size_t n3;
   double *v, *eta; /* host copies */
   hipStream_t s;
-  /* weight table: FP64 master copy and derived forms */
-  double *dW64;
-  void *dWf[NFORMS]; /* [0] aliases dW64; [1] float (forms 1, 3); [2] (hi,lo) pairs + row scales */
-  void *dWp[3][9];   /* tables permuted for the streaming kernel, per storage form and rows per lane */
-  int have_w;
+  /* weight tables: one slot per species pair of a multi-species run; t is the
+   * selected slot, on which the table calls and evaluations act */
+  struct wtable {
+    double *dW64;      /* FP64 master copy */
+    void *dWf[NFORMS]; /* [0] aliases dW64; [1] float (forms 1, 3); [2] (hi,lo) pairs + row scales */
+    void *dWp[3][9];   /* tables permuted for the streaming kernel, per storage form and rows per lane */
+    int have_w;
+  } tab[SPBTE_MAX_NS * SPBTE_MAX_NS], *t;
   /* conservation */
   int Ns;
   double masses[SPBTE_MAX_NS];
@@ -243,6 +246,8 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SPBTE_ERR_NODEV, "no GPU visible to HIP");
   spbte_op *op = calloc(1, sizeof *op);
+  if (!op) return fail(SPBTE_ERR_ARG, "out of host memory");
+  op->t = &op->tab[0];
   op->N = N;
   op->n3 = (size_t)N * N * N;
   op->v = malloc(sizeof(double) * N);
@@ -288,18 +293,30 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
 }
 
 static void free_weights(spbte_op *op) {
-  if (op->dW64) hipFree(op->dW64);
-  for (int f = 1; f < NFORMS; f++)
-    if (op->dWf[f]) hipFree(op->dWf[f]);
-  op->dW64 = NULL;
-  for (int f = 0; f < NFORMS; f++) op->dWf[f] = NULL;
-  for (int f = 0; f < 3; f++)
-    for (int t = 0; t < 9; t++) {
-      if (op->dWp[f][t]) hipFree(op->dWp[f][t]);
-      op->dWp[f][t] = NULL;
-    }
-  op->have_w = 0;
+  for (int k = 0; k < SPBTE_MAX_NS * SPBTE_MAX_NS; k++) {
+    struct wtable *w = &op->tab[k];
+    if (w->dW64) hipFree(w->dW64);
+    for (int f = 1; f < NFORMS; f++)
+      if (w->dWf[f]) hipFree(w->dWf[f]);
+    w->dW64 = NULL;
+    for (int f = 0; f < NFORMS; f++) w->dWf[f] = NULL;
+    for (int f = 0; f < 3; f++)
+      for (int t = 0; t < 9; t++) {
+        if (w->dWp[f][t]) hipFree(w->dWp[f][t]);
+        w->dWp[f][t] = NULL;
+      }
+    w->have_w = 0;
+  }
 }
+
+int spbte_op_select_table(spbte_op *op, int slot) {
+  if (!op || slot < 0 || slot >= SPBTE_MAX_NS * SPBTE_MAX_NS)
+    return fail(SPBTE_ERR_ARG, "table slot must be 0 to %d", SPBTE_MAX_NS * SPBTE_MAX_NS - 1);
+  op->t = &op->tab[slot];
+  return SPBTE_OK;
+}
+
+int spbte_op_table_slot(spbte_op *op) { return (int)(op->t - op->tab); }
 
 static void free_scratch(spbte_op *op) {
   if (op->d_mom) hipFree(op->d_mom);
@@ -356,25 +373,25 @@ static int ensure_scratch(spbte_op *op, int ncell) {
 /* ------------------------------------------------------------------ weights */
 static int alloc_w64(spbte_op *op) {
   op->gen++;
-  if (op->dW64) {
+  if (op->t->dW64) {
     HIPCHK(hipStreamSynchronize(op->s));
     for (int f = 1; f < NFORMS; f++)
-      if (op->dWf[f]) { hipFree(op->dWf[f]); op->dWf[f] = NULL; }
+      if (op->t->dWf[f]) { hipFree(op->t->dWf[f]); op->t->dWf[f] = NULL; }
     for (int f = 0; f < 3; f++)
       for (int t = 0; t < 9; t++)
-        if (op->dWp[f][t]) { hipFree(op->dWp[f][t]); op->dWp[f][t] = NULL; }
+        if (op->t->dWp[f][t]) { hipFree(op->t->dWp[f][t]); op->t->dWp[f][t] = NULL; }
     return SPBTE_OK;
   }
-  HIPCHK(hipMalloc((void **)&op->dW64, sizeof(double) * op->n3 * op->n3));
-  op->dWf[0] = op->dW64;
+  HIPCHK(hipMalloc((void **)&op->t->dW64, sizeof(double) * op->n3 * op->n3));
+  op->t->dWf[0] = op->t->dW64;
   return SPBTE_OK;
 }
 
 int spbte_op_set_weights(spbte_op *op, const double *W) {
   int rc = alloc_w64(op);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(op->dW64, W, sizeof(double) * op->n3 * op->n3, hipMemcpyHostToDevice));
-  op->have_w = 1;
+  HIPCHK(hipMemcpy(op->t->dW64, W, sizeof(double) * op->n3 * op->n3, hipMemcpyHostToDevice));
+  op->t->have_w = 1;
   return SPBTE_OK;
 }
 
@@ -387,14 +404,14 @@ int spbte_op_set_weights_rows(spbte_op *op, const double *const *rows) {
   for (size_t r0 = 0; r0 < n3; r0 += stage_rows) {
     size_t nr = (r0 + stage_rows <= n3) ? stage_rows : n3 - r0;
     for (size_t r = 0; r < nr; r++) memcpy(stage + r * n3, rows[r0 + r], sizeof(double) * n3);
-    hipError_t e = hipMemcpy(op->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(op->t->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       hipHostFree(stage);
       return fail(SPBTE_ERR_HIP, "weight upload: %s", hipGetErrorString(e));
     }
   }
   hipHostFree(stage);
-  op->have_w = 1;
+  op->t->have_w = 1;
   return SPBTE_OK;
 }
 
@@ -416,7 +433,7 @@ int spbte_op_load_weights_file(spbte_op *op, const char *path) {
       fclose(fp);
       return fail(SPBTE_ERR_IO, "weight file %s is short", path);
     }
-    hipError_t e = hipMemcpy(op->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(op->t->dW64 + r0 * n3, stage, sizeof(double) * n3 * nr, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       hipHostFree(stage);
       fclose(fp);
@@ -425,7 +442,7 @@ int spbte_op_load_weights_file(spbte_op *op, const char *path) {
   }
   hipHostFree(stage);
   fclose(fp);
-  op->have_w = 1;
+  op->t->have_w = 1;
   return SPBTE_OK;
 }
 
@@ -437,12 +454,12 @@ int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, doubl
   HIPCHK(hipMemcpy(d_eta, op->eta, sizeof(double) * op->N, hipMemcpyHostToDevice));
   const double dfac = 0.25 * pow(0.5 * (d_i + d_j), 2); /* src/weights.c:277 */
   const double mu = m_j / (m_i + m_j);                  /* src/weights.c:190 */
-  int e = spbte_launch_weights_iso(&op->g, d_eta, lambda, dfac, mu, op->dW64, op->s);
+  int e = spbte_launch_weights_iso(&op->g, d_eta, lambda, dfac, mu, op->t->dW64, op->s);
   hipError_t e2 = hipStreamSynchronize(op->s);
   hipFree(d_eta);
   if (e != 0 || e2 != hipSuccess)
     return fail(SPBTE_ERR_HIP, "weight generation: %s", hipGetErrorString(e ? (hipError_t)e : e2));
-  op->have_w = 1;
+  op->t->have_w = 1;
   return SPBTE_OK;
 }
 
@@ -453,33 +470,33 @@ int spbte_op_generate_weights_landau(spbte_op *op, double lambda) {
   double *d_eta = NULL;
   HIPCHK(hipMalloc((void **)&d_eta, sizeof(double) * op->N));
   HIPCHK(hipMemcpy(d_eta, op->eta, sizeof(double) * op->N, hipMemcpyHostToDevice));
-  int e = spbte_launch_weights_landau(&op->g, d_eta, lambda, op->dW64, op->s);
+  int e = spbte_launch_weights_landau(&op->g, d_eta, lambda, op->t->dW64, op->s);
   hipError_t e2 = hipStreamSynchronize(op->s);
   hipFree(d_eta);
   if (e != 0 || e2 != hipSuccess)
     return fail(SPBTE_ERR_HIP, "Landau weight generation: %s", hipGetErrorString(e ? (hipError_t)e : e2));
-  op->have_w = 1;
+  op->t->have_w = 1;
   return SPBTE_OK;
 }
 
 int spbte_op_get_weights(spbte_op *op, double *W) {
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
   HIPCHK(hipStreamSynchronize(op->s));
-  HIPCHK(hipMemcpy(W, op->dW64, sizeof(double) * op->n3 * op->n3, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(W, op->t->dW64, sizeof(double) * op->n3 * op->n3, hipMemcpyDeviceToHost));
   return SPBTE_OK;
 }
 
 int spbte_op_get_weights_rows(spbte_op *op, double **rows) {
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
   HIPCHK(hipStreamSynchronize(op->s));
   for (size_t r = 0; r < op->n3; r++)
-    HIPCHK(hipMemcpy(rows[r], op->dW64 + r * op->n3, sizeof(double) * op->n3, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rows[r], op->t->dW64 + r * op->n3, sizeof(double) * op->n3, hipMemcpyDeviceToHost));
   return SPBTE_OK;
 }
 
 int spbte_op_store_weights_file(spbte_op *op, const char *path) {
   if (!op || !path) return fail(SPBTE_ERR_ARG, "null argument");
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
   HIPCHK(hipStreamSynchronize(op->s));
   const size_t n3 = op->n3, stage_rows = (((size_t)64 << 20) / (n3 * 8)) ? ((size_t)64 << 20) / (n3 * 8) : 1;
   double *stage = NULL;
@@ -489,7 +506,7 @@ int spbte_op_store_weights_file(spbte_op *op, const char *path) {
   int rc = fp ? SPBTE_OK : fail(SPBTE_ERR_IO, "cannot open %s for writing", path);
   for (size_t r0 = 0; r0 < n3 && !rc; r0 += stage_rows) {
     const size_t nr = (r0 + stage_rows <= n3) ? stage_rows : n3 - r0;
-    hipError_t e = hipMemcpy(stage, op->dW64 + r0 * n3, sizeof(double) * n3 * nr, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(stage, op->t->dW64 + r0 * n3, sizeof(double) * n3 * nr, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(SPBTE_ERR_HIP, "weight download: %s", hipGetErrorString(e));
     else if (fwrite(stage, sizeof(double), n3 * nr, fp) != n3 * nr) rc = fail(SPBTE_ERR_IO, "writing %s failed", path);
   }
@@ -505,22 +522,22 @@ static int storage_of(int form) { return form == SPBTE_FORM_FP32A ? SPBTE_FORM_F
 static size_t fg_bytes(int form) { return form == SPBTE_FORM_FP32A ? 8 : 16; }
 
 static int weights_form(spbte_op *op, int form, const void **W, const float **rowscale) {
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   *rowscale = NULL;
   const int st = storage_of(form);
   if (st == SPBTE_FORM_FP64) {
-    *W = op->dW64;
+    *W = op->t->dW64;
     return SPBTE_OK;
   }
   const size_t n = op->n3 * op->n3;
-  if (!op->dWf[st]) {
+  if (!op->t->dWf[st]) {
     size_t bytes = (st == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
-    HIPCHK(hipMalloc(&op->dWf[st], bytes));
+    HIPCHK(hipMalloc(&op->t->dWf[st], bytes));
     op->gen++;
-    LCHK(spbte_launch_convert_weights(op->dW64, op->dWf[st], st, op->n3, op->n3, op->s));
+    LCHK(spbte_launch_convert_weights(op->t->dW64, op->t->dWf[st], st, op->n3, op->n3, op->s));
   }
-  *W = op->dWf[st];
-  if (st == SPBTE_FORM_FP16X2) *rowscale = (const float *)((const unsigned *)op->dWf[st] + n);
+  *W = op->t->dWf[st];
+  if (st == SPBTE_FORM_FP16X2) *rowscale = (const float *)((const unsigned *)op->t->dWf[st] + n);
   return SPBTE_OK;
 }
 
@@ -531,13 +548,13 @@ static int weights_permuted(spbte_op *op, int form, int tr, const void **Wp, con
   const void *W;
   int rc = weights_form(op, form, &W, rowscale); /* FP16x2: the row scales come from here */
   if (rc) return rc;
-  if (!op->dWp[st][tr]) {
+  if (!op->t->dWp[st][tr]) {
     const size_t n = op->n3 * op->n3;
-    HIPCHK(hipMalloc(&op->dWp[st][tr], n * (st == SPBTE_FORM_FP64 ? 8 : 4)));
+    HIPCHK(hipMalloc(&op->t->dWp[st][tr], n * (st == SPBTE_FORM_FP64 ? 8 : 4)));
     op->gen++;
-    LCHK(spbte_launch_permute_weights(op->dW64, op->dWp[st][tr], *rowscale, st, tr, op->n3, op->s));
+    LCHK(spbte_launch_permute_weights(op->t->dW64, op->t->dWp[st][tr], *rowscale, st, tr, op->n3, op->s));
   }
-  *Wp = op->dWp[st][tr];
+  *Wp = op->t->dWp[st][tr];
   return SPBTE_OK;
 }
 
@@ -693,7 +710,7 @@ static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_
 
 static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
                     int ncell) {
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   spbte_tile t;
   int have = pick_tile(op, ncell, form, nprod, &t);
   if (!have && form != SPBTE_FORM_FP64)
@@ -757,7 +774,7 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
 int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags) {
   if (!op || ncell < 0 || form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ncell == 0) return SPBTE_OK;
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   const int ch = chunk_cells(op);
   int rc = ensure_scratch(op, ncell < ch ? ncell : ch);
   if (rc) return rc;
@@ -775,6 +792,22 @@ int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
   if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
   if (ncell <= 0) return SPBTE_OK;
   LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
+  return SPBTE_OK;
+}
+
+int spbte_conserve_pairs(spbte_op *op, double *d_Q, int ncell) {
+  if (!op || !d_Q || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ncell == 0) return SPBTE_OK;
+  LCHK(spbte_launch_project_strided(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ncell, op->n3,
+                                    (size_t)ncell * op->n3, op->s));
+  return SPBTE_OK;
+}
+
+int spbte_op_get_species(spbte_op *op, int *Ns, double *masses) {
+  if (!op || !Ns) return fail(SPBTE_ERR_ARG, "null argument");
+  *Ns = op->Ns;
+  if (masses)
+    for (int m = 0; m < op->Ns; m++) masses[m] = op->masses[m];
   return SPBTE_OK;
 }
 
@@ -871,7 +904,7 @@ static const spbte_tile k_cand[] = {
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");
-  if (!op->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   const int np = maxpreserve ? 2 : 1;
   int rc = ensure_scratch(op, ncell);
   if (rc) return rc;

@@ -78,6 +78,8 @@ int spbte_launch_permute_weights(const double *W64, void *dst, const float *rows
                                  hipStream_t s);
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                          double *Q, int ncell, hipStream_t s);
+int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
+                                 double *Q, int ngroups, size_t gstride, size_t pstride, hipStream_t s);
 int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t n_rows, size_t n_cols, hipStream_t s);
 int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
                              double *W, hipStream_t s);
@@ -98,6 +100,7 @@ int spbte_launch_lincomb(size_t n, double a, const double *x, double b, const do
 int spbte_launch_out_quantities(const spbte_grid_dev *g, const double *f, int ncell, double *out, hipStream_t s);
 /* run internals used by the output module (spbte_run.c) */
 struct spbte_run;
+int spbte_run_nspecies(struct spbte_run *run);
 int spbte_run_geometry(struct spbte_run *run, struct spbte_op **op, int *dim, int *ncell, const double **d_state,
                        const double **d_x);
 
@@ -143,6 +146,7 @@ unsigned spbte_op_generation(struct spbte_op *op);
 const void *spbte_op_qhat_buffer(struct spbte_op *op);
 int spbte_op_chunk_cells(struct spbte_op *op);
 int spbte_op_n(struct spbte_op *op);
+int spbte_op_table_slot(struct spbte_op *op);
 int spbte_fail(int code, const char *fmt, ...);
 
 #ifdef __cplusplus

@@ -17,19 +17,20 @@
 #include "spbte.h"
 #include "spbte_internal.h"
 
-#define MAX_RUNS_OPEN 16
+#define MAX_FILES_OPEN 64
 
 typedef struct {
   spbte_run *run;
+  int species;
   FILE *fp;
   int flags;
 } out_slot;
 
-static out_slot slots[MAX_RUNS_OPEN];
+static out_slot slots[MAX_FILES_OPEN];
 
-static out_slot *slot_of(spbte_run *run) {
-  for (int i = 0; i < MAX_RUNS_OPEN; i++)
-    if (slots[i].run == run && slots[i].fp) return &slots[i];
+static out_slot *slot_of(spbte_run *run, int species) {
+  for (int i = 0; i < MAX_FILES_OPEN; i++)
+    if (slots[i].run == run && slots[i].fp && slots[i].species == species) return &slots[i];
   return NULL;
 }
 
@@ -65,11 +66,12 @@ int spbte_read_output_flags(const char *path, int *flags) {
   return SPBTE_OK;
 }
 
-int spbte_run_open_output(spbte_run *run, const char *path, int flags, int append) {
+int spbte_run_open_output_species(spbte_run *run, int species, const char *path, int flags, int append) {
   if (!run || !path) return spbte_fail(SPBTE_ERR_ARG, "null argument");
-  if (slot_of(run)) return spbte_fail(SPBTE_ERR_ARG, "the run already has an open output file");
+  if (species < 0 || species >= spbte_run_nspecies(run)) return spbte_fail(SPBTE_ERR_ARG, "no species %d", species);
+  if (slot_of(run, species)) return spbte_fail(SPBTE_ERR_ARG, "species %d already has an open output file", species);
   out_slot *sl = NULL;
-  for (int i = 0; i < MAX_RUNS_OPEN && !sl; i++)
+  for (int i = 0; i < MAX_FILES_OPEN && !sl; i++)
     if (!slots[i].fp) sl = &slots[i];
   if (!sl) return spbte_fail(SPBTE_ERR_ARG, "too many open output files");
   spbte_op *op;
@@ -103,26 +105,33 @@ int spbte_run_open_output(spbte_run *run, const char *path, int flags, int appen
     fprintf(fp, "\n");
   }
   sl->run = run;
+  sl->species = species;
   sl->fp = fp;
   sl->flags = flags;
   return SPBTE_OK;
 }
 
-int spbte_run_write_output(spbte_run *run, double time) {
-  out_slot *sl = slot_of(run);
-  if (!sl) return spbte_fail(SPBTE_ERR_ARG, "the run has no open output file");
+int spbte_run_open_output(spbte_run *run, const char *path, int flags, int append) {
+  return spbte_run_open_output_species(run, 0, path, flags, append);
+}
+
+/* Lines of one species at one time into its file. */
+static int write_species(out_slot *sl, spbte_run *run, double time) {
   spbte_op *op;
   int dim, ncell;
-  const double *d_state, *d_x;
-  int rc = spbte_run_geometry(run, &op, &dim, &ncell, &d_state, &d_x);
+  const double *d_state0, *d_x;
+  int rc = spbte_run_geometry(run, &op, &dim, &ncell, &d_state0, &d_x);
+  if (rc) return rc;
+  double *d_state = NULL;
+  rc = spbte_run_species_state(run, sl->species, &d_state);
   if (rc) return rc;
   const spbte_grid_dev *g = spbte_op_grid(op);
-  const int N = g->N, per = 2 + N;
-  double *mom = malloc(sizeof(double) * 5 * ncell), *q = malloc(sizeof(double) * per * ncell);
+  const int N = g->N, per = 2 + N, ns = spbte_run_nspecies(run);
+  double *mom = malloc(sizeof(double) * 5 * ncell * ns), *q = malloc(sizeof(double) * per * ncell);
   double *x = dim == 1 ? malloc(sizeof(double) * ncell) : NULL;
   double *d_q = NULL;
   if (!mom || !q || (dim == 1 && !x)) rc = spbte_fail(SPBTE_ERR_ARG, "out of host memory");
-  if (!rc) rc = spbte_run_moments(run, mom); /* rho, u_x, u_y, u_z, T; synchronous */
+  if (!rc) rc = spbte_run_moments(run, mom); /* per species rho, u_x, u_y, u_z, T; synchronous */
   if (!rc && dim == 0) {
     if (hipMalloc((void **)&d_q, sizeof(double) * per * ncell) != hipSuccess)
       rc = spbte_fail(SPBTE_ERR_HIP, "output buffer allocation failed");
@@ -134,11 +143,14 @@ int spbte_run_write_output(spbte_run *run, double time) {
   }
   if (!rc && dim == 1 && hipMemcpy(x, d_x, sizeof(double) * ncell, hipMemcpyDeviceToHost) != hipSuccess)
     rc = spbte_fail(SPBTE_ERR_HIP, "mesh copy failed");
-  /* species mass m: rho = m int f, u unchanged, T = (m / KB) times the
-   * mass-1 temperature (src/momentRoutines.c:58-183); exact for m = KB = 1 */
+  /* species-0 mass m for every species, as src/output.c:171-177 and
+   * :311-313 call the moment routines with species 0: rho = m int f, u
+   * unchanged, T = (m / KB) times the mass-1 temperature
+   * (src/momentRoutines.c:58-183); exact for m = KB = 1 */
   const double m = g->mass, tfac = g->mass / g->kb;
+  const double *ms = mom + (size_t)5 * ncell * sl->species;
   for (int c = 0; c < ncell && !rc; c++) {
-    const double rho = m * mom[5 * c], ux = mom[5 * c + 1], T = tfac * mom[5 * c + 4];
+    const double rho = m * ms[5 * c], ux = ms[5 * c + 1], T = tfac * ms[5 * c + 4];
     FILE *fp = sl->fp;
     fprintf(fp, "%le ", time);
     if (dim == 1) fprintf(fp, "%le ", x[c]);
@@ -162,32 +174,77 @@ int spbte_run_write_output(spbte_run *run, double time) {
   return rc;
 }
 
-int spbte_run_close_output(spbte_run *run) {
-  out_slot *sl = slot_of(run);
-  if (!sl) return spbte_fail(SPBTE_ERR_ARG, "the run has no open output file");
-  int rc = fclose(sl->fp) == 0 ? SPBTE_OK : spbte_fail(SPBTE_ERR_IO, "closing the output file failed");
-  sl->fp = NULL;
-  sl->run = NULL;
-  return rc;
+int spbte_run_write_output(spbte_run *run, double time) {
+  int any = 0;
+  for (int l = 0; l < spbte_run_nspecies(run); l++) {
+    out_slot *sl = slot_of(run, l);
+    if (!sl) continue;
+    any = 1;
+    int rc = write_species(sl, run, time);
+    if (rc) return rc;
+  }
+  return any ? SPBTE_OK : spbte_fail(SPBTE_ERR_ARG, "the run has no open output file");
 }
 
-int spbte_run_store_restart(spbte_run *run, const char *pdf_path, const char *time_path, int step) {
+int spbte_run_close_output(spbte_run *run) {
+  int any = 0, rc = SPBTE_OK;
+  for (int i = 0; i < MAX_FILES_OPEN; i++)
+    if (slots[i].fp && slots[i].run == run) {
+      any = 1;
+      if (fclose(slots[i].fp) != 0 && !rc) rc = spbte_fail(SPBTE_ERR_IO, "closing an output file failed");
+      slots[i].fp = NULL;
+      slots[i].run = NULL;
+    }
+  return any ? rc : spbte_fail(SPBTE_ERR_ARG, "the run has no open output file");
+}
+
+/* One species' interior cells to or from a raw dump (src/restart.c:41-54, :80-95). */
+static int species_dump(spbte_run *run, int species, const char *pdf_path, int load) {
   if (!run || !pdf_path) return spbte_fail(SPBTE_ERR_ARG, "null argument");
   spbte_op *op;
   int dim, ncell;
-  const double *d_state, *d_x;
-  int rc = spbte_run_geometry(run, &op, &dim, &ncell, &d_state, &d_x);
+  const double *d_state0, *d_x;
+  int rc = spbte_run_geometry(run, &op, &dim, &ncell, &d_state0, &d_x);
+  if (rc) return rc;
+  double *d_state = NULL;
+  rc = spbte_run_species_state(run, species, &d_state);
   if (rc) return rc;
   const size_t n = (size_t)ncell * spbte_op_n(op) * spbte_op_n(op) * spbte_op_n(op);
   double *f = malloc(sizeof(double) * n);
   if (!f) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
-  rc = spbte_run_get_state(run, f);
-  FILE *fp = rc ? NULL : fopen(pdf_path, "wb");
-  if (!rc && !fp) rc = spbte_fail(SPBTE_ERR_IO, "cannot open %s", pdf_path);
-  if (!rc && (fwrite(f, sizeof(double), n, fp) != n || fflush(fp) != 0))
-    rc = spbte_fail(SPBTE_ERR_IO, "writing %s failed", pdf_path);
+  if (hipStreamSynchronize(spbte_op_stream(op)) != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "stream synchronisation failed");
+  FILE *fp = NULL;
+  if (!rc && !load) {
+    if (hipMemcpy(f, d_state, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = spbte_fail(SPBTE_ERR_HIP, "state copy failed");
+    fp = rc ? NULL : fopen(pdf_path, "wb");
+    if (!rc && !fp) rc = spbte_fail(SPBTE_ERR_IO, "cannot open %s", pdf_path);
+    if (!rc && (fwrite(f, sizeof(double), n, fp) != n || fflush(fp) != 0))
+      rc = spbte_fail(SPBTE_ERR_IO, "writing %s failed", pdf_path);
+  } else if (!rc) {
+    fp = fopen(pdf_path, "rb");
+    if (!fp) rc = spbte_fail(SPBTE_ERR_IO, "Error: unable to open restarted file %s", pdf_path);
+    size_t got = rc ? 0 : fread(f, sizeof(double), n, fp);
+    if (!rc && got != n)
+      rc = spbte_fail(SPBTE_ERR_IO, "Error reloading pdf file %s (%zu of %zu values)", pdf_path, got, n);
+    if (!rc && hipMemcpy(d_state, f, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+      rc = spbte_fail(SPBTE_ERR_HIP, "state copy failed");
+  }
   if (fp) fclose(fp);
   free(f);
+  return rc;
+}
+
+int spbte_run_store_restart_species(spbte_run *run, int species, const char *pdf_path) {
+  return species_dump(run, species, pdf_path, 0);
+}
+
+int spbte_run_load_restart_species(spbte_run *run, int species, const char *pdf_path) {
+  return species_dump(run, species, pdf_path, 1);
+}
+
+int spbte_run_store_restart(spbte_run *run, const char *pdf_path, const char *time_path, int step) {
+  int rc = species_dump(run, 0, pdf_path, 0);
   if (!rc && time_path) {
     FILE *ft = fopen(time_path, "wb");
     if (!ft) return spbte_fail(SPBTE_ERR_IO, "cannot open %s", time_path);
@@ -199,22 +256,7 @@ int spbte_run_store_restart(spbte_run *run, const char *pdf_path, const char *ti
 }
 
 int spbte_run_load_restart(spbte_run *run, const char *pdf_path, const char *time_path, int *step) {
-  if (!run || !pdf_path) return spbte_fail(SPBTE_ERR_ARG, "null argument");
-  spbte_op *op;
-  int dim, ncell;
-  const double *d_state, *d_x;
-  int rc = spbte_run_geometry(run, &op, &dim, &ncell, &d_state, &d_x);
-  if (rc) return rc;
-  const size_t n = (size_t)ncell * spbte_op_n(op) * spbte_op_n(op) * spbte_op_n(op);
-  FILE *fp = fopen(pdf_path, "rb");
-  if (!fp) return spbte_fail(SPBTE_ERR_IO, "Error: unable to open restarted file %s", pdf_path);
-  double *f = malloc(sizeof(double) * n);
-  if (!f) { fclose(fp); return spbte_fail(SPBTE_ERR_ARG, "out of host memory"); }
-  const size_t got = fread(f, sizeof(double), n, fp);
-  fclose(fp);
-  if (got != n) { free(f); return spbte_fail(SPBTE_ERR_IO, "Error reloading pdf file %s (%zu of %zu values)", pdf_path, got, n); }
-  rc = spbte_run_set_state(run, f);
-  free(f);
+  int rc = species_dump(run, 0, pdf_path, 1);
   if (!rc && time_path && step) {
     FILE *ft = fopen(time_path, "rb");
     if (!ft) return spbte_fail(SPBTE_ERR_IO, "cannot open %s", time_path);

@@ -188,8 +188,9 @@ def test_boltz_main_errors_before_any_gpu_call(spb, tmp_path):
     with pytest.raises(spb.SpbteError, match="input file not found"):
         spb.boltz("absent.in", "absent.out", workdir=tmp_path)
     (tmp_path / "input").mkdir()
-    (tmp_path / "input" / "two.in").write_text("Init_field\n2\nTime_step\n0.01\nnum_species\n2\ndefault\ndefault\nStop\n")
-    with pytest.raises(spb.SpbteError, match="one species"):
+    (tmp_path / "input" / "two.in").write_text(
+        "Init_field\n2\nTime_step\n0.01\nAnisotropic\n1\nnum_species\n2\ndefault\ndefault\nStop\n")
+    with pytest.raises(spb.SpbteError, match="Anisotropic 1 with 2 species"):
         spb.boltz("two.in", "x.out", workdir=tmp_path)
     (tmp_path / "input" / "nodt.in").write_text("Init_field\n2\nStop\n")
     with pytest.raises(spb.SpbteError, match="Time_step"):

@@ -315,3 +315,57 @@ def test_landau_entry_closed_form_and_quadrature(orc, lam):
         ref = 2 * math.pi * integrate.dblquad(f, 0, Lv, 0, math.pi, epsabs=1e-12, epsrel=1e-12)[0]
         got = orc.landau_entry(zeta, xi, Lv, lam)
         assert abs(got - ref) < 1e-10 * abs(ref), (got, ref)
+
+
+def test_cross_species_maxwellians_are_an_equilibrium(orc):
+    """Two species at a common temperature, Maxwellians of their own masses,
+    are a collision equilibrium: with the table of the pair built as
+    exec/boltz.c:157-160 and src/weights.c:190 orient it (species_i the first
+    argument's, mu = m_j / (m_i + m_j)), Q(M_0, M_1) vanishes to the spectral
+    error; with the masses swapped it does not.  Pins the mass orientation of
+    orc_weights_iso and the argument roles of orc_compute_q for unequal masses."""
+    N, Lv = 16, 6.0
+    v, eta = orc.grid_0d(N, Lv)
+    V2 = v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2
+    M = lambda m: (m / (2 * np.pi)) ** 1.5 * np.exp(-m * V2 / 2)  # noqa: E731
+    right = orc.weights_iso(N, eta, Lv, 1.0, 2.0, 2.0, 1.0, 2.0)
+    swapped = orc.weights_iso(N, eta, Lv, 1.0, 2.0, 2.0, 2.0, 1.0)
+    q_right = np.abs(orc.compute_q(N, v, eta, right, M(1.0), M(2.0))).max()
+    q_swapped = np.abs(orc.compute_q(N, v, eta, swapped, M(1.0), M(2.0))).max()
+    assert q_right < 2e-4 and q_swapped > 1e-2, (q_right, q_swapped)
+
+
+def test_two_identical_species_equal_one_species_at_half_knudsen(orc):
+    """With two copies of one species (same mass, same table, same state) every
+    pair's Q is Q(f, f), the joint projection splits evenly, and each copy
+    advances as one species with Kn / 2: pins the pair sums and the update
+    order of the multi-species drivers (exec/boltz.c:193-241, :285-354)."""
+    N, Lv, dt, Kn = 8, 5.0, 0.01, 1.0
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_iso(N, eta, Lv, 1.0)
+    f0 = orc.bkw_initial(N, v).ravel()
+    two = orc.run_0d_species(N, v, eta, [W] * 4, [f0, f0], [1.0, 1.0], dt, 3, order=2, Kn=Kn)
+    one = orc.run_0d(N, v, eta, W, f0, dt, 3, order=2, Kn=Kn / 2)
+    states = []
+    orc_f = f0.copy()
+    cons = orc.Conserver(N, v)
+    for _ in range(3):  # the same single-species Heun step, returning states
+        Q = cons(orc.compute_q_maxpreserve(N, v, eta, W, orc_f, orc_f).ravel())
+        f1 = orc_f + dt * Q / (Kn / 2)
+        Q = cons(orc.compute_q_maxpreserve(N, v, eta, W, f1, f1).ravel())
+        orc_f = 0.5 * (orc_f + f1) + 0.5 * dt * Q / (Kn / 2)
+        states.append(orc_f.copy())
+    assert len(one) == 4
+    for t in range(3):
+        for l in range(2):
+            assert np.abs(two[t][l] - states[t]).max() < 1e-13 * np.abs(states[t]).max()
+    x, dx = orc.uniform_mesh(6, 1.0, 1)
+    v1, eta1 = orc.grid_1d(N, Lv)
+    W1 = orc.weights_iso(N, eta1, Lv, 1.0)
+    g0 = np.stack([orc.maxwellian_1d(v1, 1.0 + 0.1 * c, 0.2, 1.0 + 0.05 * c).ravel() for c in range(6)])
+    two = orc.run_1d_species(N, v1, eta1, [W1] * 4, [g0, g0], [1.0, 1.0], x, dx, 0.002, 2, 1, Kn, 1)
+    ref = []
+    orc.run_1d(N, v1, eta1, W1, g0, x, dx, 0.002, 2, 1, Kn / 2, 1, callback=lambda t, f: ref.append(f[1:7].copy()))
+    for t in range(2):
+        for l in range(2):
+            assert np.abs(two[t][l] - ref[t]).max() < 1e-13 * np.abs(ref[t]).max()
