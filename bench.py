@@ -187,6 +187,8 @@ def main():
     if not a.no_extra and not a.no_runs:
         extra["shock_run"] = shock_run(spb, a.form)
         extra["ensemble_mixed_run"] = mixed_ensemble_run(spb, a.form)
+    if not a.no_extra and not a.no_runs:
+        extra["landau_table"] = landau_table(spb)
     if not a.no_extra and not a.no_surrogate:
         extra["surrogate"] = surrogate_run(spb, a.form)
 
@@ -234,6 +236,24 @@ def shock_run(spb, form, steps=3):
                         "Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
             "form": form, "ms_per_step": round(ms, 3), "q_per_s": round(2 * nX / ms * 1e3, 1),
             "steps_timed": steps, "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": ok}
+
+
+def landau_table(spb, N=32, Lv=7.0, lam=1.0):
+    """The anisotropic generator's glance-0 (Landau) table on the GPU, on the
+    1D grid of the heat-transfer configuration (N=32, L_v=7, lambda=1):
+    N^6 = 1.07e9 entries, each 64 x 64 Gauss-Legendre nodes (4096 J0 and
+    sincos evaluations)."""
+    from spectralbte_amd import workloads
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    t0 = time.perf_counter()
+    op.generate_weights_landau(lam)
+    sec = time.perf_counter() - t0
+    op.close()
+    n = float(N) ** 6
+    log(f"Landau table N={N}: {sec:.2f} s, {n / sec:.3g} entries/s")
+    return {"workload": f"Landau (glance 0) weight table, N={N}, L_v={Lv}, lambda={lam}, 1D grid", "seconds": round(sec, 2),
+            "entries_per_s": round(n / sec, 1), "integrand_evals_per_s": round(n * 4096 / sec, 1)}
 
 
 def _mesh(nX, length, order):
