@@ -139,3 +139,45 @@ def test_output_flags_file(spb, tmp_path):
     p.write_text("density\n1\n")
     with pytest.raises(spb.SpbteError):
         spb.read_output_flags(str(p))
+
+
+def _aniso_call(spb, N, eta, Lv, lam, weight_flag, glance, wd):
+    L = spb.lib()
+    cw = ctypes.POINTER(ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))()
+    L.alloc_weights(N, ctypes.byref(cw), 1)
+    L.initialize_weights_AnIso.restype = None
+    L.initialize_weights_AnIso.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
+    eta_c = np.ascontiguousarray(eta)
+    cwd = os.getcwd()
+    os.chdir(wd)
+    try:
+        L.initialize_weights_AnIso(N, eta_c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), Lv, lam, weight_flag,
+                                   ctypes.cast(cw[0], ctypes.c_void_p), glance)
+    finally:
+        os.chdir(cwd)
+    return cw[0], eta_c
+
+
+def test_initialize_weights_aniso_loads_landau_file_and_ghat3l(spb, orc, tmp_path):
+    """initialize_weights_AnIso (src/aniso_weights.c:37-81) with glance 0 finds
+    Weights/N4_Aniso_L_v5_lambda1_Landau.wts and loads it into the caller's rows
+    without a GPU (the third argument is L_v, as exec/boltz.c:154 passes it);
+    gHat3L (:371-398) then matches the oracle's entry."""
+    N, Lv, lam = 4, 5.0, 1.0
+    v, eta = orc.grid_0d(N, Lv)
+    W = orc.weights_landau(N, eta, Lv, lam)
+    (tmp_path / "Weights").mkdir()
+    W.tofile(tmp_path / "Weights" / "N4_Aniso_L_v5_lambda1_Landau.wts")
+    rows, eta_c = _aniso_call(spb, N, eta, Lv, lam, 0, 0.0, tmp_path)
+    for r in (0, 21, 63):
+        assert np.array_equal(np.ctypeslib.as_array(rows[r], shape=(64,)), W[r])
+    L = spb.lib()
+    L.gHat3L.restype = ctypes.c_double
+    L.gHat3L.argtypes = [ctypes.c_double] * 6
+    for (i, j, k), (l, m, n) in [((0, 1, 2), (3, 0, 1)), ((2, 2, 2), (1, 3, 0)), ((3, 3, 3), (2, 2, 2))]:
+        got = L.gHat3L(eta[i], eta[j], eta[k], eta[l], eta[m], eta[n])
+        ref = orc.landau_entry((eta[i], eta[j], eta[k]), (eta[l], eta[m], eta[n]), Lv, lam)
+        assert abs(got - ref) <= 1e-14 * abs(ref)
+        assert abs(got - W[(i * N + j) * N + k, (l * N + m) * N + n]) <= 1e-14 * abs(ref)
+    L.dealloc_weights(N, rows)

@@ -349,3 +349,35 @@ def test_weights_landau_matches_oracle(spb, orc, N, Lv, lam, rows):
     want = -(8 * np.pi / 3) * z2 * Lv ** (lam + 5) / (lam + 5)
     assert np.abs(W[:, c0] - want).max() <= 1e-13 * np.abs(want).max()
     op.close()
+
+
+def test_initialize_weights_aniso_generates_landau_rows(spb, orc, tmp_path):
+    """initialize_weights_AnIso with glance 0 and no file generates the Landau
+    table on the GPU into the caller's rows (src/aniso_weights.c:82-106) and
+    stores it under the reference's name."""
+    import ctypes
+    N, Lv, lam = 8, 5.0, 0.0
+    v, eta = orc.grid_0d(N, Lv)
+    (tmp_path / "Weights").mkdir()
+    L = spb.lib()
+    cw = ctypes.POINTER(ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))()
+    L.alloc_weights(N, ctypes.byref(cw), 1)
+    L.initialize_weights_AnIso.restype = None
+    L.initialize_weights_AnIso.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
+    eta_c = np.ascontiguousarray(eta)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        L.initialize_weights_AnIso(N, eta_c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), Lv, lam, 0,
+                                   ctypes.cast(cw[0], ctypes.c_void_p), 0.0)
+    finally:
+        os.chdir(cwd)
+    rows = cw[0]
+    stored = np.fromfile(tmp_path / "Weights" / "N8_Aniso_L_v5_lambda0_Landau.wts").reshape(512, 512)
+    for r in (5, 300):
+        ref = orc.weights_landau(N, eta, Lv, lam, rows=(r, 1))[0]
+        got = np.ctypeslib.as_array(rows[r], shape=(512,))
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert np.array_equal(stored[r], got)
+    L.dealloc_weights(N, rows)
