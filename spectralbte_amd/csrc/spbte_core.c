@@ -925,8 +925,16 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     if (!tile_supported(op->N, form, np, t)) continue;
     if (t->tb > 2 * ncell) continue;
     if (t->kind > 1 && stream_wgs(op->N, ncell, t) >= 2048) continue; /* the chip is full without a split */
-    rc = launch_tile(op, form, t, &args); /* warm-up; builds the table layout it needs */
+    /* warm-up (builds the table layout it needs); a tile whose warm-up alone
+     * takes twice the best time so far is not timed further */
+    HIPCHK(hipEventRecord(a, op->s));
+    rc = launch_tile(op, form, t, &args);
+    HIPCHK(hipEventRecord(b, op->s));
+    HIPCHK(hipEventSynchronize(b));
     if (rc) break;
+    float warm = 0.f;
+    HIPCHK(hipEventElapsedTime(&warm, a, b));
+    if (warm > 2.0f * best_ms) continue;
     HIPCHK(hipEventRecord(a, op->s));
     const int reps = 2;
     for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args);
