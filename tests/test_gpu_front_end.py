@@ -201,3 +201,24 @@ def test_boltz_landau_table_generated_and_used(spb, orc, tmp_path):
     spb.weightgen(N, 0, 1, 1.0, Lv, workdir=g)
     assert np.array_equal(np.fromfile(g / "Weights" / name), np.fromfile(d / "Weights" / name))
     spb.weightgen(N, 0, 1, 1.0, Lv, workdir=g)  # an existing table is kept
+
+
+@pytest.mark.parametrize("N", [9, 12])
+def test_boltz_other_grids_vs_oracle(spb, orc, tmp_path, N):
+    """The front end on grids other than the fixtures' (odd N uses the 0D grid's
+    0.5 (N-1) h_eta half-width, src/initializer.c:74-78): a BKW run with the
+    isotropic Maxwell-molecule table generated on the GPU against the oracle's
+    run with the stored table."""
+    Lv, dt, nT = 5.0, 0.01, 3
+    text = _input_text(N=N, L_v=Lv, Knudsen=1.0, Lambda=0.0, Time_step=dt, Number_of_time_steps=nT, Space_order=2,
+                       Data_writing_frequency=1, Init_field=2, SpaceInhom=0)
+    d = _workdir(tmp_path, f"n{N}", {"run.in": text, "run.out": os.path.join(GOLDEN, "BKW8", "BKW8.test.out")})
+    spb.boltz("run.in", "run.out", workdir=d)
+    W = orc.weights_load(str(d / "Weights" / f"N{N}_isotropic_L_v5_lambda0.wts"), N)
+    v, eta = orc.grid_0d(N, Lv)
+    assert np.abs(W - orc.weights_iso(N, eta, Lv, 0.0)).max() <= 1e-13 * np.abs(W).max()
+    ref = np.array(orc.run_0d(N, v, eta, W, orc.bkw_initial(N, v).ravel(), dt, nT, order=2))
+    got = _rows(d / "Data" / "moments_run.in")
+    assert got.shape == ref.shape
+    m = np.abs(ref) > 1e-12
+    assert (np.abs(got - ref)[m] / np.abs(ref)[m]).max() < 1e-6
