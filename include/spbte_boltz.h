@@ -2,8 +2,8 @@
  *
  * The reference's driver (exec/boltz.c) reads an input file of keywords, an
  * optional mesh file and species files, builds the grids and the initial
- * state, and advances one species in time with the moment output of
- * src/output.c and the restart dumps of src/restart.c.  spbte_boltz_main does
+ * state, and advances one or several species in time with the moment output
+ * of src/output.c and the restart dumps of src/restart.c.  spbte_boltz_main does
  * the same with every time step on the GPU (spbte_run0d / spbte_run1d), and
  * the pieces are exported for hosts that build their own driver.
  *

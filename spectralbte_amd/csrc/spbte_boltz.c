@@ -4,9 +4,10 @@
  * (src/mesh_setup.c:9-200) and species files (src/species.c:28-160), builds
  * the velocity and Fourier grids and the initial state (src/initializer.c),
  * loads, or generates on the GPU and stores, the weight table
- * (src/weights.c:65-124), and runs the time loop of exec/boltz.c:181-398 for
- * one species as whole steps on the GPU, with the moment output of
- * src/output.c and the restart dumps of src/restart.c.
+ * (src/weights.c:65-124; the Landau table, src/aniso_weights.c:37-146), and
+ * runs the time loop of exec/boltz.c:181-398 for one or several species as
+ * whole steps on the GPU, with the moment output of src/output.c and the
+ * restart dumps of src/restart.c.
  */
 #include <math.h>
 #include <stdio.h>
