@@ -35,6 +35,15 @@ def log(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
+def guarded(name, fn, *args):
+    """fn(*args), or {"error": ...} with the failure logged."""
+    try:
+        return fn(*args)
+    except Exception as e:  # noqa: BLE001 -- any failure of a secondary measurement
+        log(f"{name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -172,7 +181,7 @@ def main():
                       "fp64_plain_w_stream_gbs": round(8.0 * P / (kp * 1e-3) / 1e9, 1),
                       "speedup_vs_plain_fp64": round(value / plain, 2)})
 
-    if not a.no_extra:
+    def secondary():
         other = "shock" if a.workload == "ensemble" else "ensemble"
         wl2 = Workload(spb, other, 256 if other == "shock" else 512)
         ms2, mk2, t2, Q2 = wl2.run(a.form, 3, 1)
@@ -181,20 +190,24 @@ def main():
             ms3, _, t3, Q3 = wl2.run("fp64", 2, 1)
             sec.update({"fp64_batched_q_per_s": round(wl2.ncell / (ms3 * 1e-3), 1), "fp64_tile": list(t3)})
             sec.update(wl2.errors(Q2, Q3))
-        extra["secondary"] = sec
         wl2.op.close()
+        return sec
 
+    # the secondary measurements report an error instead of costing the headline line
+    if not a.no_extra:
+        extra["secondary"] = guarded("secondary workload", secondary)
     if not a.no_extra and not a.no_runs:
-        extra["shock_run"] = shock_run(spb, a.form)
-        extra["ensemble_mixed_run"] = mixed_ensemble_run(spb, a.form)
-    if not a.no_extra and not a.no_runs:
-        extra["landau_table"] = landau_table(spb)
+        extra["shock_run"] = guarded("shock run", shock_run, spb, a.form)
+        extra["ensemble_mixed_run"] = guarded("mixed ensemble run", mixed_ensemble_run, spb, a.form)
+        extra["landau_table"] = guarded("Landau table", landau_table, spb)
     if not a.no_extra and not a.no_surrogate:
-        extra["surrogate"] = surrogate_run(spb, a.form)
+        extra["surrogate"] = guarded("surrogate", surrogate_run, spb, a.form)
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(wl, a.cpu_cells)
+        cpu = guarded("CPU baseline", cpu_baseline, wl, a.cpu_cells)
+        if "error" in cpu:
+            cpu = None
 
     out = {"metric": "Q(f) evals/s at N=32 (batched FP32 vs plain FP64) at <=1e-6 rel err; HBM GB/s",
            "value": round(value, 1), "unit": "Q evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
