@@ -44,16 +44,19 @@ struct SGeom {
   // window refill of every step of a body reads at (base - j + BS) with j the
   // step's constant offset in the body, and never wraps
   static constexpr int LW = N + BS;
-  // Line storage.  When every lane's window start is a multiple of 4 (TR and
-  // N/2 multiples of 4), entry m is stored at (m mod 4) * LW/4 + m / 4: the
-  // refills of the lanes of one z-line (entries 4 apart) then fall on
-  // consecutive 16-byte slots instead of one slot in four of the banks, and a
-  // refill offset stays a compile-time constant per body step.  Otherwise
-  // entries are stored in order.
-  static constexpr bool TL = (TR % 4 == 0) && ((N / 2) % 4 == 0);
-  static constexpr int LS = LW / 4;
-  static_assert(!TL || LW % 4 == 0, "transposed lines need LW % 4 == 0");
-  __host__ __device__ static constexpr int pos(int m) { return TL ? (m & 3) * LS + (m >> 2) : m; }
+  // Line storage.  When every lane's window start is a multiple of G = 4 (TR
+  // and N/2 multiples of 4) or of G = 3 (TR, N/2 and LW multiples of 3),
+  // entry m is stored at (m mod G) * LW/G + m / G: the refills of the lanes of
+  // one z-line (entries TR apart, a multiple of G) then fall on consecutive
+  // 16-byte slots instead of one slot in G of the banks, and a refill offset
+  // stays a compile-time constant per body step.  Otherwise entries are
+  // stored in order (G = 1).
+  static constexpr int G = ((TR % 4 == 0) && ((N / 2) % 4 == 0) && (LW % 4 == 0))
+                               ? 4
+                               : (((TR % 3 == 0) && ((N / 2) % 3 == 0) && (LW % 3 == 0)) ? 3 : 1);
+  static constexpr bool TL = G > 1;
+  static constexpr int LS = LW / G;
+  __host__ __device__ static constexpr int pos(int m) { return TL ? (m % G) * LS + m / G : m; }
   static_assert(N % BS == 0, "cz body must divide N");
   static_assert(N % TR == 0, "TR must divide N");
   static_assert((N * N * N) % RWG == 0, "workgroups must tile the output rows");
@@ -219,9 +222,10 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
               for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int b = 0; b < TB; ++b) {
-                  // entry BS + base - j: with transposed lines BS + base = 4 B4 + 3
-                  // and fl points at slot B4 of sub-row 0
-                  if constexpr (Gm::TL) buf[p][b][slnew] = fl[p][b][((3 - j) & 3) * Gm::LS + ((3 - j) >> 2)];
+                  // entry BS + base - j: with transposed lines BS + base = G BG + G - 1
+                  // and fl points at slot BG of sub-row 0
+                  if constexpr (Gm::TL)
+                    buf[p][b][slnew] = fl[p][b][(Gm::G - 1 - j % Gm::G) * Gm::LS - j / Gm::G];
                   else buf[p][b][slnew] = fl[p][b][-j];
                 }
             }
@@ -238,7 +242,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-          for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + ((BS + base) >> 2) : line[p][b] + BS + base;
+          for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + (BS + base) / Gm::G : line[p][b] + BS + base;
         const double2 *gz = gline + it * BS;
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
