@@ -97,6 +97,15 @@ int spbte_op_generate_weights_iso(spbte_op *op, double lambda, double d_i, doubl
  * 64-point Gauss-Legendre rules, on the GPU.  As the reference writes it, the
  * table carries no trapezoid or h_eta^3 factors. */
 int spbte_op_generate_weights_landau(spbte_op *op, double lambda);
+/* The anisotropic generator's glancing-collision table, glance > 0: gHat3
+ * (exec/MPIcollisionRoutines.c:33-203, the table exec/MPIWeightGenerator.c
+ * writes as Weights/N%d_AnIso_L_v%g_lambda%g_glance%g.wts) with the eps-linear
+ * cross section (8/pi) glance / theta^4 on [sqrt(glance), pi] and the Taylor
+ * term for [0, sqrt(glance)], at every (row, column) pair, on the GPU.  The
+ * reference's nested adaptive rules (tolerance 1e-6) are replaced by a reduced
+ * form of the same integral computed to about 1e-10.  0 < glance < pi, else
+ * SPBTE_ERR_ARG. */
+int spbte_op_generate_weights_glance(spbte_op *op, double lambda, double glance);
 int spbte_op_get_weights(spbte_op *op, double *W);
 /* Writes the FP64 table as a Weights/ file (N^3 rows of N^3 doubles, the
  * layout src/weights.c:95-104 stores). */

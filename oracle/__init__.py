@@ -27,6 +27,7 @@ _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
 
 
 def build(force=False):
@@ -63,6 +64,8 @@ def lib():
             "orc_set_wall_kb": (None, [D]),
             "orc_landau_entry": (D, [D, D, D, D, D, D, D, D]),
             "orc_weights_landau_rows": (None, [I, _dp, D, D, ctypes.c_long, ctypes.c_long, _dp]),
+            "orc_glance_entry": (D, [D, D, D, D, D, D, D, D, D, I, I, I, I]),
+            "orc_weights_glance_entries": (None, [I, _dp, D, D, D, ctypes.c_long, _lp, _lp, I, I, I, I, _dp, I]),
             "orc_weights_load": (I, [ctypes.c_char_p, I, _dp]),
             "orc_diffuse_bc": (None, [I, _dp, D, _dp, _dp, D, I]),
             "orc_upwind_one": (None, [I, I, _dp, D, _dp, D, I, D, D, _dp, _dp]),
@@ -79,6 +82,10 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(_dp)
+
+
+def _lpp(a):
+    return a.ctypes.data_as(_lp)
 
 
 def _c(a, dtype=np.float64):
@@ -209,6 +216,24 @@ def weights_landau(N, eta, Lv, lam, rows=None):
     W = np.zeros((nr, N ** 3))
     lib().orc_weights_landau_rows(N, _p(_c(eta)), Lv, lam, r0, nr, _p(W))
     return W
+
+
+def glance_entry(zeta, xi, Lv, lam, glance, panels=(4, 4, 3, 3)):
+    """gHat3 (exec/MPIcollisionRoutines.c:149-203) for row mode zeta, column mode
+    xi: fixed composite 16-point rules with (r, phi, log-theta, theta) panel
+    counts `panels` in place of the reference's adaptive GSL rules."""
+    return lib().orc_glance_entry(*map(float, zeta), *map(float, xi), float(Lv), float(lam), float(glance),
+                                  *map(int, panels))
+
+
+def weights_glance_entries(N, eta, Lv, lam, glance, rows, cols, panels=(4, 4, 3, 3), nthreads=None):
+    """Entries (rows[e], cols[e]) of the glance > 0 table."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.zeros(len(rows))
+    lib().orc_weights_glance_entries(N, _p(_c(eta)), Lv, lam, glance, len(rows), _lpp(rows), _lpp(cols),
+                                     *map(int, panels), _p(out), nthreads or default_threads())
+    return out
 
 
 def weights_load(path, N):

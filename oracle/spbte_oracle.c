@@ -630,6 +630,138 @@ void orc_weights_landau_rows(int N, const double *eta, double Lv, double lam, lo
   }
 }
 
+/* ---- Glancing-collision table (glance > 0), gHat3 of
+ * exec/MPIcollisionRoutines.c:149-203 with its integrands ghat_theta (:33-57,
+ * the eps-linear cross section b = (8/pi) glance / theta^4), ghat_phi (:78-121,
+ * the Taylor term C (2/pi) log(glance) / log(sin(glance/2)) for the excluded
+ * [0, sqrt(glance)] part) and ghat_r (:124-140):
+ *   W = 4 pi^2 int_0^L r^(lam+2) int_0^pi sin(phi) J0(r sin(phi) xi_perp) (R1 + R2) dphi dr,
+ *   R1 = int_sqrt(g)^pi b(theta) [cos(a7 (1 - cos theta) - a6) J0(a8 sin theta) - cos a6] dtheta,
+ *   a6 = r cos(phi) xi.zeta/|zeta|, a7 = r cos(phi) |zeta| / 2, a8 = r sin(phi) |zeta| / 2.
+ * The reference integrates adaptively (gsl_integration_qag, 61-point
+ * Gauss-Kronrod, for r and phi; gsl_integration_cquad for theta; absolute and
+ * relative tolerance 1e-6 each).  Here every level is a fixed composite rule
+ * of 16-point Gauss-Legendre panels: pr panels on [0, L], pphi on [0, pi];
+ * theta in log(theta) on [sqrt(g), 1] (pt1 panels; the integrand grows like
+ * 1/theta^2 towards the lower end) and in theta on [max(1, sqrt(g)), pi] (pt2
+ * panels), so the sum converges to the integral the reference approximates. */
+static void gl16_half(double *x, double *w) {
+  const int n = 16;
+  for (int i = 0; i < n / 2; i++) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L)), pp = 0.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p1 = 1.0L, p2 = 0.0L;
+      for (int j = 1; j <= n; j++) {
+        long double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0L);
+      long double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) < 1e-19L) break;
+    }
+    x[n / 2 - 1 - i] = (double)z;
+    w[n / 2 - 1 - i] = (double)(2.0L / ((1.0L - z * z) * pp * pp));
+  }
+}
+
+/* nodes and weights of P composite 16-point panels on [a, b] */
+static int composite16(double a, double b, int P, const double *gx, const double *gw, double *x, double *w) {
+  const double h = (b - a) / P;
+  int n = 0;
+  for (int p = 0; p < P; p++) {
+    const double c = a + h * (p + 0.5);
+    for (int i = 0; i < 8; i++) {
+      x[n] = c - 0.5 * h * gx[i];
+      w[n++] = 0.5 * h * gw[i];
+      x[n] = c + 0.5 * h * gx[i];
+      w[n++] = 0.5 * h * gw[i];
+    }
+  }
+  return n;
+}
+
+double orc_glance_entry(double z1, double z2, double z3, double x1, double x2, double x3, double Lv, double lam,
+                        double glance, int pr, int pphi, int pt1, int pt2) {
+  double gx[8], gw[8];
+  gl16_half(gx, gw);
+  /* geometry as gHat3 (:165-183) */
+  const double zetalen2 = z1 * z1 + z2 * z2 + z3 * z3;
+  const double xilen2 = x1 * x1 + x2 * x2 + x3 * x3;
+  const double xizeta = x1 * z1 + x2 * z2 + x3 * z3;
+  const double zetalen = sqrt(zetalen2);
+  double xiperp;
+  if (((xilen2 - xizeta * xizeta / zetalen2) < 0) || (zetalen2 == 0)) xiperp = 0;
+  else xiperp = sqrt(xilen2 - xizeta * xizeta / zetalen2);
+  const double arg0 = zetalen, arg1 = (zetalen != 0) ? xizeta / zetalen : 0.0, arg2 = xiperp;
+
+  /* theta rule, with b(theta) and the Jacobian folded into the weights */
+  const double t0 = sqrt(glance);
+  const int nt_max = 16 * (pt1 + pt2);
+  double *tx = malloc(sizeof(double) * nt_max), *tw = malloc(sizeof(double) * nt_max);
+  double *ct = malloc(sizeof(double) * nt_max), *st = malloc(sizeof(double) * nt_max);
+  int nt = 0;
+  if (t0 < 1.0) {
+    nt = composite16(log(t0), 0.0, pt1, gx, gw, tx, tw);
+    for (int k = 0; k < nt; k++) {
+      tx[k] = exp(tx[k]);
+      tw[k] *= tx[k];
+    }
+  }
+  if (t0 < M_PI) nt += composite16(t0 > 1.0 ? t0 : 1.0, M_PI, pt2, gx, gw, tx + nt, tw + nt);
+  for (int k = 0; k < nt; k++) {
+    const double th = tx[k];
+    tw[k] *= (8.0 / M_PI) * (glance / (th * th)) * pow(th, -2.0); /* ghat_theta :51 */
+    ct[k] = cos(th);
+    st[k] = sin(th);
+  }
+  const double Lfac = (2.0 / M_PI) * log(glance) / log(sin(0.5 * glance)); /* :119 */
+
+  double *rx = malloc(sizeof(double) * 16 * pr), *rw = malloc(sizeof(double) * 16 * pr);
+  double *px = malloc(sizeof(double) * 16 * pphi), *pw = malloc(sizeof(double) * 16 * pphi);
+  const int nr = composite16(0.0, Lv, pr, gx, gw, rx, rw);
+  const int np = composite16(0.0, M_PI, pphi, gx, gw, px, pw);
+  double sr = 0.0;
+  for (int i = 0; i < nr; i++) {
+    const double r = rx[i];
+    double sp = 0.0;
+    for (int j = 0; j < np; j++) {
+      const double arg4 = cos(px[j]), arg5 = sin(px[j]); /* ghat_phi :97-102 */
+      const double arg6 = r * arg4 * arg1, arg7 = 0.5 * r * arg4 * arg0, arg8 = 0.5 * r * arg5 * arg0;
+      const double arg9 = cos(r * arg4 * arg1);
+      double R1 = 0.0;
+      for (int k = 0; k < nt; k++) /* ghat_theta :55 */
+        R1 += tw[k] * (cos(arg7 * (1 - ct[k]) - arg6) * j0(arg8 * st[k]) - arg9);
+      const double c1 = 0.5 * r * arg0 * arg4, c2 = 0.5 * r * arg0 * arg5, c3 = r * arg1 * arg4;
+      const double C = (-0.25 * c2 * c2 * cos(c3) + 0.5 * c1 * sin(c3));
+      sp += pw[j] * (arg5 * j0(r * arg5 * arg2) * (R1 + C * Lfac)); /* :121 */
+    }
+    sr += rw[i] * (pow(r, lam + 2) * sp); /* ghat_r :139 */
+  }
+  free(tx); free(tw); free(ct); free(st); free(rx); free(rw); free(px); free(pw);
+  return 4 * M_PI * M_PI * sr; /* :202 */
+}
+
+/* generate_conv_weights with glance > 0 (exec/MPIcollisionRoutines.c:251-281)
+ * at n chosen entries: out[e] = gHat3(eta_row, eta_col) for row rows[e] and
+ * column cols[e] (row = output mode zeta, column = summed mode xi).  OpenMP
+ * over entries. */
+void orc_weights_glance_entries(int N, const double *eta, double Lv, double lam, double glance, long n,
+                                const long *rows, const long *cols, int pr, int pphi, int pt1, int pt2, double *out,
+                                int nthreads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (long e = 0; e < n; e++) {
+    const long r = rows[e], c = cols[e];
+    const int i = (int)(r / ((long)N * N)), j = (int)((r / N) % N), k = (int)(r % N);
+    const int l = (int)(c / ((long)N * N)), m = (int)((c / N) % N), nn = (int)(c % N);
+    out[e] = orc_glance_entry(eta[i], eta[j], eta[k], eta[l], eta[m], eta[nn], Lv, lam, glance, pr, pphi, pt1, pt2);
+  }
+  (void)nthreads;
+}
+
 /* src/weights.c:68,78-88: raw native-endian FP64, N^3 rows of N^3, no header.
  * Returns 0 on success, -1 if the file is missing, -2 if it is short. */
 int orc_weights_load(const char *path, int N, double *W) {

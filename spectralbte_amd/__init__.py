@@ -143,6 +143,7 @@ def lib():
             "spbte_run_set_forcing": (I, [_vp, D]),
             "spbte_op_store_weights_file": (I, [_vp, ctypes.c_char_p]),
             "spbte_op_generate_weights_landau": (I, [_vp, D]),
+            "spbte_op_generate_weights_glance": (I, [_vp, D, D]),
             "spbte_op_select_table": (I, [_vp, I]),
             "spbte_conserve_pairs": (I, [_vp, _vp, I]),
             "spbte_op_get_species": (I, [_vp, ip, _dp]),
@@ -277,6 +278,10 @@ class Operator:
     def generate_weights_landau(self, lam):
         """The anisotropic generator's glance = 0 (Landau) table on this grid."""
         _check(lib().spbte_op_generate_weights_landau(self.h, float(lam)))
+
+    def generate_weights_glance(self, lam, glance):
+        """The anisotropic generator's glancing-collision (glance > 0) table on this grid."""
+        _check(lib().spbte_op_generate_weights_glance(self.h, float(lam), float(glance)))
 
     def store_weights_file(self, path):
         _check(lib().spbte_op_store_weights_file(self.h, os.fsencode(path)))

@@ -4,8 +4,16 @@
  * Replaces src/aniso_weights.c for boltz_: initialize_weights_AnIso
  * (:37-146) with its file name, load-or-generate rule and messages, gHat3L
  * (:371-398) as a scalar host function, and generate_conv_weights_AnIso
- * (:400-428) through spbte_op_generate_weights_landau (k_weights.hip).  The
- * glancing-collision table (glance > 0) is refused.
+ * (:400-428) through spbte_op_generate_weights_landau (k_weights.hip).
+ *
+ * With glance > 0 the reference's file-scope glance (:31), which selects
+ * gHat3_AnIso in generate_conv_weights_AnIso (:417), is never assigned:
+ * initialize_weights_AnIso's parameter of the same name only picks the file
+ * name (:65-68).  A missing glance file is therefore filled with the Landau
+ * table under the glance name, and so it is here.  The glancing-collision
+ * table itself is what exec/MPIWeightGenerator.c writes
+ * (spbte_weightgen_main, spbte_op_generate_weights_glance); present, it is
+ * loaded.
  */
 #include <math.h>
 #include <stdio.h>
@@ -22,7 +30,7 @@
 
 static int N;
 static double *eta; /* borrowed */
-static double L_v, lambda_, glance_;
+static double L_v, lambda_;
 static double gl_x[32], gl_w[32]; /* 64-point Gauss-Legendre: positive nodes ascending, weights */
 static int gl_ready;
 
@@ -93,10 +101,6 @@ static void die(const char *what) {
 }
 
 void generate_conv_weights_AnIso(double **conv_weights) {
-  if (glance_ != 0.0) {
-    printf("The glancing-collision table (glance %g) is not generated here; glance 0 gives the Landau table\n", glance_);
-    exit(1);
-  }
   double *v = malloc(sizeof(double) * N);
   if (!v) die("generate_conv_weights_AnIso");
   for (int i = 0; i < N; i++) v[i] = -L_v + i * (2 * L_v / (N - 1));
@@ -130,7 +134,6 @@ void initialize_weights_AnIso(int nodes, double *zeta, double Lv, double lam, in
   eta = zeta;
   L_v = Lv;
   lambda_ = lam;
-  glance_ = glance;
   gl64();
   const size_t n3 = (size_t)N * N * N;
   for (size_t i = 0; i < n3; i++) conv_weights[i] = malloc(n3 * sizeof(double));

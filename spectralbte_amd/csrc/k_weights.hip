@@ -1,7 +1,8 @@
 // k_weights.hip -- convolution weight tables generated on the GPU: the
 // isotropic table (reference: src/weights.c:265-281 generate_conv_weights_iso
 // with gHat3, :181-262, and the integrand ghat, :156-160) and the Landau
-// table of the anisotropic generator with glance = 0 (further below).
+// table of the anisotropic generator with glance = 0, and its glancing-
+// collision table with glance > 0 (further below).
 //
 //   W[r][c] = w_c1 w_c2 w_c3 * dfac * P * I_lam(mu|eta_r|, |eta_c|, |eta_c - mu eta_r|)
 //   I_lam(a0,a1,a2) = int_0^L r^(lam+2) [sinc(a0 r) sinc(a2 r) - sinc(a1 r)] dr
@@ -13,6 +14,9 @@
 // composite 64-panel x 20-point Gauss-Legendre rule per entry.
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <vector>
+
 #include "spbte_internal.h"
 
 namespace {
@@ -233,6 +237,261 @@ extern "C" int spbte_launch_weights_landau(const spbte_grid_dev *g, const double
     if (e != hipSuccess) return (int)e;
   }
   return 0;
+}
+
+// ---- Glancing-collision table (glance g > 0): gHat3 of
+// exec/MPIcollisionRoutines.c:149-203, the table exec/MPIWeightGenerator.c
+// writes.  The reference nests three adaptive quadratures per entry,
+//   W = 4 pi^2 int_0^L r^(lam+2) int_0^pi sin(phi) J0(r sin(phi) xi_perp) (R1 + R2) dphi dr,
+//   R1 = int_sqrt(g)^pi b(theta) [cos(a7 (1 - cos theta) - a6) J0(a8 sin theta) - cos a6] dtheta,
+//   R2 = C(r, phi) (2/pi) Lam,  Lam = log g / log sin(g/2),  b(theta) = (8/pi) g / theta^4,
+// (a6 = r cos(phi) a1, a7 = r cos(phi) |zeta|/2, a8 = r sin(phi) |zeta|/2,
+// a1 = xi.zeta/|zeta|; C as in ghat_phi :111-114).  The same integral is
+// evaluated here in a reduced form.  With J0(x s) J0(y s) = (1/pi) int_0^pi
+// J0(s sqrt(x^2 + y^2 - 2 x y cos psi)) dpsi (Neumann's addition theorem) and
+// int_0^pi sin(phi) cos(X cos phi) J0(Z sin phi) dphi = 2 sinc(sqrt(X^2 + Z^2))
+// (the sphere average of a plane wave), the phi integral of R1 is a psi
+// average of sincs, and the r integral of a sinc is one function D of one
+// variable:
+//   R1 part = 8 pi^2 int_sqrt(g)^pi b(theta) [ (1/pi) int_0^pi D(kappa) dpsi - D(|xi|) ] dtheta,
+//   kappa^2 = |xi|^2 + (|zeta|^2 - 2 xi.zeta) sin^2(theta/2) - xi_perp |zeta| sin(theta) cos(psi),
+//   D(k) = int_0^L r^(lam+2) (sinc(k r) - 1) dr.
+// The R2 part is Lam/4 times the exact Landau integral, whose phi integral is
+// a combination of derivatives of the same sinc (S = sinc):
+//   R2 part = (pi Lam / 2) [2 |zeta|^2 (c_perp^2 E1(q) + (2 - c_perp^2) E2(q)) - 8 |zeta| c_par E3(q)],
+//   q = |xi|, c_perp = xi_perp / q, c_par = a1 / q,
+//   E1 = int r^(lam+4) S''(q r) dr,  E2 = int r^(lam+4) S'(q r)/(q r) dr,  E3 = int r^(lam+3) S'(q r) dr.
+// D, E1, E2, E3 are tabulated once per table on the host (piecewise Chebyshev,
+// degree 15, panels of width 3/L; values from composite Gauss-Legendre sums in
+// r), so an entry costs a 2-D sum of Chebyshev evaluations instead of a 3-D
+// sum of Bessel functions.  theta: 16-point panels in log(theta) on
+// [sqrt(g), 1] and in theta on [max(1, sqrt(g)), pi]; psi: the trapezoid rule,
+// exact to rounding for the smooth even periodic integrand.
+namespace {
+
+struct GlanceDims {
+  int nt, np;              // theta nodes, psi nodes
+  int nD, nE;              // Chebyshev panels of D and of E1..E3
+  double invhD, invhE;     // 1 / panel widths
+  double lam4;             // pi Lam / 2
+  int total;               // doubles in the rule buffer
+};
+constexpr int kChebN = 16, kChebStride = 17;  // coefficients per panel, padded stride (LDS banks)
+
+__device__ __forceinline__ double cheb_eval(const double *tab, int P, double invh, double x) {
+  const double u = x * invh;
+  int p = (int)u;
+  p = p < 0 ? 0 : (p >= P ? P - 1 : p);
+  const double t = 2.0 * (u - p) - 1.0, t2 = 2.0 * t;
+  const double *c = tab + p * kChebStride;
+  double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+  for (int k = kChebN - 1; k >= 1; --k) {
+    const double b0 = fma(t2, b1, c[k] - b2);
+    b2 = b1;
+    b1 = b0;
+  }
+  return fma(t, b1, c[0] - b2);
+}
+
+// rule buffer: s2[nt] = sin^2(theta/2), sth[nt] = sin(theta), wt[nt] = theta
+// weight x b(theta) x 8 pi^2; cps[np] = cos(psi), wps[np] = psi weight / pi;
+// D table (nD panels), E1, E2, E3 tables (nE panels each).
+__global__ void __launch_bounds__(256) k_weights_glance(int N, const double *__restrict__ eta, long r0, long nrows,
+                                                        const double *__restrict__ rule, GlanceDims d,
+                                                        double *__restrict__ W) {
+  extern __shared__ double sm[];
+  for (int i = threadIdx.x; i < d.total; i += 256) sm[i] = rule[i];
+  __syncthreads();
+  const double *s2 = sm, *sth = s2 + d.nt, *wt = sth + d.nt, *cps = wt + d.nt, *wps = cps + d.np;
+  const double *tD = wps + d.np, *tE1 = tD + d.nD * kChebStride, *tE2 = tE1 + d.nE * kChebStride,
+               *tE3 = tE2 + d.nE * kChebStride;
+  const long n3 = (long)N * N * N;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nrows * n3) return;
+  const long r = r0 + e / n3, c = e % n3;
+  const double z1 = eta[r / ((long)N * N)], z2 = eta[(r / N) % N], z3 = eta[r % N];
+  const double x1 = eta[c / ((long)N * N)], x2 = eta[(c / N) % N], x3 = eta[c % N];
+  // geometry as gHat3 (:165-183)
+  const double zetalen2 = z1 * z1 + z2 * z2 + z3 * z3;
+  const double xilen2 = x1 * x1 + x2 * x2 + x3 * x3;
+  const double xizeta = x1 * z1 + x2 * z2 + x3 * z3;
+  const double zetalen = sqrt(zetalen2);
+  const double perp2 = xilen2 - xizeta * xizeta / zetalen2;
+  const double xp = (perp2 < 0 || zetalen2 == 0) ? 0.0 : sqrt(perp2);
+  const double a1 = zetalen != 0 ? xizeta / zetalen : 0.0;
+  const double q2 = a1 * a1 + xp * xp, q = sqrt(q2);
+  const double P = zetalen2 - 2.0 * zetalen * a1, Q = xp * zetalen;
+  const double D0 = cheb_eval(tD, d.nD, d.invhD, q);
+  double t1 = 0.0;
+  for (int i = 0; i < d.nt; ++i) {
+    const double A = fma(P, s2[i], q2), B = Q * sth[i];
+    double acc = 0.0;
+    for (int j = 0; j < d.np; ++j) {
+      const double k2 = fma(-B, cps[j], A);
+      acc = fma(wps[j], cheb_eval(tD, d.nD, d.invhD, sqrt(k2 > 0.0 ? k2 : 0.0)), acc);
+    }
+    t1 = fma(wt[i], acc - D0, t1);
+  }
+  double cperp2 = 0.0, cpar = 0.0;
+  if (q > 0.0) {
+    cperp2 = xp * xp / q2;
+    cpar = a1 / q;
+  }
+  const double E1 = cheb_eval(tE1, d.nE, d.invhE, q), E2 = cheb_eval(tE2, d.nE, d.invhE, q),
+               E3 = cheb_eval(tE3, d.nE, d.invhE, q);
+  const double t2 = d.lam4 * (2.0 * zetalen2 * (cperp2 * E1 + (2.0 - cperp2) * E2) - 8.0 * zetalen * cpar * E3);
+  W[(r - r0) * n3 + c] = t1 + t2;
+}
+
+// sinc and its derivatives; power series below |x| = 1 (no cancellation)
+enum SincKind { kSm1, kS1, kS1x, kS2 };  // S - 1, S', S'/x, S''
+double sinc_fn(SincKind k, double x) {
+  if (fabs(x) < 1.0) {
+    // S = sum_n (-1)^n x^(2n) / (2n+1)!
+    double s = 0.0, fact = 1.0, xp = 1.0;  // fact = (2n+1)!, xp = x^(2n-2)
+    for (int n = 1; n <= 12; ++n) {
+      fact *= (2.0 * n) * (2.0 * n + 1.0);
+      const double sg = (n & 1) ? -1.0 : 1.0;
+      double term;
+      switch (k) {
+        case kSm1: term = xp * x * x; break;
+        case kS1: term = 2.0 * n * xp * x; break;
+        case kS1x: term = 2.0 * n * xp; break;
+        default: term = 2.0 * n * (2.0 * n - 1.0) * xp; break;
+      }
+      s += sg * term / fact;
+      xp *= x * x;
+    }
+    return s;
+  }
+  const double sx = sin(x), cx = cos(x);
+  switch (k) {
+    case kSm1: return sx / x - 1.0;
+    case kS1: return (x * cx - sx) / (x * x);
+    case kS1x: return (x * cx - sx) / (x * x * x);
+    default: return ((2.0 - x * x) * sx - 2.0 * x * cx) / (x * x * x);
+  }
+}
+
+// int_0^L r^pw f(k r) dr by 16-point panels of width <= 2/kmax, the first
+// panel split geometrically towards r = 0 (r^pw need not be smooth there)
+struct RadialRule {
+  std::vector<double> r, w;
+  RadialRule(double L, double kmax, double pw) {
+    double gx[8], gw[8];
+    gauss_legendre_half(16, gx, gw);
+    const int P = (int)ceil(kmax * L / 2.0) + 2;
+    const double h = L / P;
+    auto panel = [&](double a, double b) {
+      const double c = 0.5 * (a + b), hh = 0.5 * (b - a);
+      for (int i = 0; i < 8; ++i)
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          const double x = c + sgn * hh * gx[i];
+          r.push_back(x);
+          w.push_back(hh * gw[i] * pow(x, pw));
+        }
+    };
+    double a = h / 1024.0;
+    panel(0.0, a);
+    for (; a < h; a *= 4.0) panel(a, fmin(4.0 * a, h));
+    for (int p = 1; p < P; ++p) panel(p * h, (p + 1) * h);
+  }
+  double sum(SincKind k, double kappa) const {
+    double s = 0.0;
+    for (size_t i = 0; i < r.size(); ++i) s += w[i] * sinc_fn(k, kappa * r[i]);
+    return s;
+  }
+};
+
+// piecewise Chebyshev coefficients of f on [0, P h], appended to out
+template <class F>
+void cheb_table(int P, double h, F f, std::vector<double> &out) {
+  double fv[kChebN];
+  for (int p = 0; p < P; ++p) {
+    for (int j = 0; j < kChebN; ++j) {
+      const double t = cos(M_PI * (j + 0.5) / kChebN);
+      fv[j] = f(h * (p + 0.5 * (t + 1.0)));
+    }
+    for (int k = 0; k < kChebN; ++k) {
+      double c = 0.0;
+      for (int j = 0; j < kChebN; ++j) c += fv[j] * cos(M_PI * k * (j + 0.5) / kChebN);
+      out.push_back((k == 0 ? 1.0 : 2.0) * c / kChebN);
+    }
+    out.push_back(0.0);  // pad to kChebStride
+  }
+}
+
+}  // namespace
+
+// Limits of the glance table: 0 < g < pi (log sin(g/2) < 0); the rule and
+// tables must fit the workgroup's LDS.
+extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double *eta_host, const double *eta,
+                                           double lam, double glance, double *W, hipStream_t s) {
+  const int N = g->N;
+  const long n3 = (long)N * N * N;
+  const double L = g->Lv;
+  double emax = 0.0;
+  for (int i = 0; i < N; ++i) emax = fmax(emax, fabs(eta_host[i]));
+  const double zmax = sqrt(3.0) * emax, kmax = 2.0 * zmax;  // |xi|, |zeta| <= zmax; kappa <= |xi| + |zeta|
+  std::vector<double> buf;
+  GlanceDims d;
+  // theta rule
+  double gx[8], gw[8];
+  gauss_legendre_half(16, gx, gw);
+  std::vector<double> th, tw;
+  auto panels = [&](double a, double b, int P, bool logv) {
+    const double h = (b - a) / P;
+    for (int p = 0; p < P; ++p)
+      for (int i = 0; i < 8; ++i)
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          const double u = a + h * (p + 0.5) + sgn * 0.5 * h * gx[i], w = 0.5 * h * gw[i];
+          th.push_back(logv ? exp(u) : u);
+          tw.push_back(logv ? w * exp(u) : w);
+        }
+  };
+  const double t0 = sqrt(glance);
+  if (t0 < 1.0) panels(log(t0), 0.0, 1 + (int)ceil(-log(t0) / 1.2), true);
+  if (t0 < M_PI) panels(fmax(t0, 1.0), M_PI, 2 + (int)ceil(L * zmax / 40.0), false);
+  d.nt = (int)th.size();
+  const int M = 8 * (int)ceil((16.0 + L * zmax / 4.0) / 8.0);  // psi intervals
+  d.np = M + 1;
+  for (int i = 0; i < d.nt; ++i) buf.push_back(sin(0.5 * th[i]) * sin(0.5 * th[i]));
+  for (int i = 0; i < d.nt; ++i) buf.push_back(sin(th[i]));
+  for (int i = 0; i < d.nt; ++i)
+    buf.push_back(8.0 * M_PI * M_PI * tw[i] * (8.0 / M_PI) * glance / (th[i] * th[i] * th[i] * th[i]));
+  for (int j = 0; j <= M; ++j) buf.push_back(cos(M_PI * j / M));
+  for (int j = 0; j <= M; ++j) buf.push_back((j == 0 || j == M ? 0.5 : 1.0) / M);
+  // radial tables
+  const double hc = 3.0 / L;
+  d.nD = (int)ceil(kmax / hc) + 1;
+  d.nE = (int)ceil(zmax / hc) + 1;
+  d.invhD = d.invhE = 1.0 / hc;
+  {
+    const RadialRule r2(L, d.nD * hc, lam + 2.0), r3(L, d.nE * hc, lam + 3.0), r4(L, d.nE * hc, lam + 4.0);
+    cheb_table(d.nD, hc, [&](double k) { return r2.sum(kSm1, k); }, buf);
+    cheb_table(d.nE, hc, [&](double k) { return r4.sum(kS2, k); }, buf);
+    cheb_table(d.nE, hc, [&](double k) { return r4.sum(kS1x, k); }, buf);
+    cheb_table(d.nE, hc, [&](double k) { return r3.sum(kS1, k); }, buf);
+  }
+  d.lam4 = 0.5 * M_PI * log(glance) / log(sin(0.5 * glance));
+  d.total = (int)buf.size();
+  const size_t lds = sizeof(double) * buf.size();
+  if (lds > 64 * 1024) return (int)hipErrorInvalidValue;
+  double *drule = NULL;
+  hipError_t e = hipMalloc((void **)&drule, lds);
+  if (e != hipSuccess) return (int)e;
+  e = hipMemcpy(drule, buf.data(), lds, hipMemcpyHostToDevice);
+  // rows per launch: about 4M entries
+  const long rows = n3 >= (4L << 20) ? 1 : (4L << 20) / n3;
+  for (long r0 = 0; e == hipSuccess && r0 < n3; r0 += rows) {
+    const long nr = r0 + rows <= n3 ? rows : n3 - r0;
+    const unsigned blocks = (unsigned)((nr * n3 + 255) / 256);
+    hipLaunchKernelGGL(k_weights_glance, dim3(blocks), dim3(256), lds, s, N, eta, r0, nr, drule, d, W + r0 * n3);
+    e = hipGetLastError();
+  }
+  const hipError_t e2 = hipStreamSynchronize(s), e3 = hipFree(drule);
+  return e != hipSuccess ? (int)e : (e2 != hipSuccess ? (int)e2 : (int)e3);
 }
 
 extern "C" int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,

@@ -399,14 +399,18 @@ static int setup_weights(spbte_op *op, const spbte_input *in, const species *si,
   return spbte_op_store_weights_file(op, name);
 }
 
-/* exec/MPIWeightGenerator.c main on one GPU: N glance beta lambda L_v.  The
- * grid is the generator's own (h_eta = 2 pi / (N h_v), L_eta = N h_eta / 2,
- * :63-74); an existing file is kept (:87-90).  glance = 0 writes the Landau
- * table; the glancing-collision table (glance > 0, nested adaptive GSL
- * quadrature, exec/MPIcollisionRoutines.c:33-188) is not built. */
+/* exec/MPIWeightGenerator.c main on one GPU: N glance beta lambda L_v (beta
+ * is read and unused, as there).  The grid is the generator's own (h_eta =
+ * 2 pi / (N h_v), L_eta = N h_eta / 2, :63-74); an existing file is kept
+ * (:87-90).  glance = 0 writes the Landau table to
+ * Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts, glance > 0 the glancing-
+ * collision table (exec/MPIcollisionRoutines.c:33-203) to
+ * Weights/N%d_AnIso_L_v%g_lambda%g_glance%g.wts (:80-83). */
 int spbte_weightgen_main(int argc, char **argv) {
   if (argc < 6 || !argv) {
-    fprintf(stderr, "usage: %s N glance beta lambda L_v   (writes Weights/N<N>_Aniso_L_v<L_v>_lambda<lambda>_Landau.wts)\n",
+    fprintf(stderr,
+            "usage: %s N glance beta lambda L_v   (writes Weights/N<N>_Aniso_L_v<L_v>_lambda<lambda>_Landau.wts for "
+            "glance 0, Weights/N<N>_AnIso_L_v<L_v>_lambda<lambda>_glance<glance>.wts otherwise)\n",
             (argc > 0 && argv && argv[0]) ? argv[0] : "weightgen_amd");
     return 1;
   }
@@ -415,13 +419,11 @@ int spbte_weightgen_main(int argc, char **argv) {
   int rc = SPBTE_OK;
   if (N < 2 || !(L_v > 0) || !isfinite(lambda))
     rc = spbte_fail(SPBTE_ERR_ARG, "need N >= 2, L_v > 0 and a finite lambda");
-  else if (glance != 0)
-    rc = spbte_fail(SPBTE_ERR_UNSUPPORTED,
-                    "glance %g: only the Landau table (glance 0) is generated here; the glancing-collision table uses "
-                    "nested adaptive quadrature (exec/MPIcollisionRoutines.c:33-188) and is not built",
-                    glance);
+  else if (glance != 0 && !(glance > 0 && glance < M_PI))
+    rc = spbte_fail(SPBTE_ERR_ARG, "glance %g outside (0, pi), where log(sin(glance/2)) < 0", glance);
   char name[512];
-  snprintf(name, sizeof name, "Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts", N, L_v, lambda);
+  if (glance == 0) snprintf(name, sizeof name, "Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts", N, L_v, lambda);
+  else snprintf(name, sizeof name, "Weights/N%d_AnIso_L_v%g_lambda%g_glance%g.wts", N, L_v, lambda, glance);
   FILE *fp = rc ? NULL : fopen(name, "rb");
   if (fp) {
     fclose(fp);
@@ -440,7 +442,8 @@ int spbte_weightgen_main(int argc, char **argv) {
     printf("Stored weights not found, generating...\n");
     rc = spbte_op_create(&op, N, L_v, v, eta);
   }
-  if (!rc) rc = spbte_op_generate_weights_landau(op, lambda);
+  if (!rc) rc = glance == 0 ? spbte_op_generate_weights_landau(op, lambda)
+                            : spbte_op_generate_weights_glance(op, lambda, glance);
   if (!rc) rc = spbte_op_store_weights_file(op, name);
   spbte_op_destroy(op);
   free(v);

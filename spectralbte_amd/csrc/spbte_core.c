@@ -479,6 +479,24 @@ int spbte_op_generate_weights_landau(spbte_op *op, double lambda) {
   return SPBTE_OK;
 }
 
+int spbte_op_generate_weights_glance(spbte_op *op, double lambda, double glance) {
+  if (!op || !isfinite(lambda) || !isfinite(glance)) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (!(glance > 0.0 && glance < M_PI))
+    return fail(SPBTE_ERR_ARG, "glance %g outside (0, pi), where log(sin(glance/2)) < 0", glance);
+  int rc = alloc_w64(op);
+  if (rc) return rc;
+  double *d_eta = NULL;
+  HIPCHK(hipMalloc((void **)&d_eta, sizeof(double) * op->N));
+  HIPCHK(hipMemcpy(d_eta, op->eta, sizeof(double) * op->N, hipMemcpyHostToDevice));
+  int e = spbte_launch_weights_glance(&op->g, op->eta, d_eta, lambda, glance, op->t->dW64, op->s);
+  hipError_t e2 = hipStreamSynchronize(op->s);
+  hipFree(d_eta);
+  if (e != 0 || e2 != hipSuccess)
+    return fail(SPBTE_ERR_HIP, "glancing-collision weight generation: %s", hipGetErrorString(e ? (hipError_t)e : e2));
+  op->t->have_w = 1;
+  return SPBTE_OK;
+}
+
 int spbte_op_get_weights(spbte_op *op, double *W) {
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table");
   HIPCHK(hipStreamSynchronize(op->s));

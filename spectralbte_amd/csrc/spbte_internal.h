@@ -84,6 +84,8 @@ int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t 
 int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
                              double *W, hipStream_t s);
 int spbte_launch_weights_landau(const spbte_grid_dev *g, const double *eta, double lam, double *W, hipStream_t s);
+int spbte_launch_weights_glance(const spbte_grid_dev *g, const double *eta_host, const double *eta, double lam,
+                                double glance, double *W, hipStream_t s);
 
 /* transport and time-integration kernels (k_transport.hip) */
 int spbte_launch_upwind1(int N, int nX, const double *f, double *fc, const double *v, const double *dx, double dt,

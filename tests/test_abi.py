@@ -159,6 +159,21 @@ def _aniso_call(spb, N, eta, Lv, lam, weight_flag, glance, wd):
     return cw[0], eta_c
 
 
+def test_initialize_weights_aniso_loads_glance_file(spb, orc, tmp_path):
+    """With glance > 0, initialize_weights_AnIso loads
+    Weights/N4_AnIso_L_v5_lambda1_glance0.01.wts (src/aniso_weights.c:68,
+    :71-81) without a GPU."""
+    N, Lv, lam = 4, 5.0, 1.0
+    v, eta = orc.grid_0d(N, Lv)
+    W = np.random.default_rng(4).standard_normal((64, 64))
+    (tmp_path / "Weights").mkdir()
+    W.tofile(tmp_path / "Weights" / "N4_AnIso_L_v5_lambda1_glance0.01.wts")
+    rows, _ = _aniso_call(spb, N, eta, Lv, lam, 0, 0.01, tmp_path)
+    for r in (0, 33, 63):
+        assert np.array_equal(np.ctypeslib.as_array(rows[r], shape=(64,)), W[r])
+    spb.lib().dealloc_weights(N, rows)
+
+
 def test_initialize_weights_aniso_loads_landau_file_and_ghat3l(spb, orc, tmp_path):
     """initialize_weights_AnIso (src/aniso_weights.c:37-81) with glance 0 finds
     Weights/N4_Aniso_L_v5_lambda1_Landau.wts and loads it into the caller's rows

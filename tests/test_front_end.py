@@ -206,12 +206,14 @@ def test_boltz_executable_usage():
     assert r.returncode == 1 and "usage" in r.stderr
 
 
-def test_weightgen_refuses_glancing_tables_and_prints_usage(spb, tmp_path):
-    """glance > 0 (the nested adaptive GSL quadrature) is refused before any GPU
+def test_weightgen_refuses_bad_glance_and_prints_usage(spb, tmp_path):
+    """glance outside (0, pi), where log(sin(glance/2)) of
+    exec/MPIcollisionRoutines.c:119 is not negative, is refused before any GPU
     call; without arguments the executable prints its usage."""
     (tmp_path / "Weights").mkdir()
-    with pytest.raises(spb.SpbteError, match="glance 0.1"):
-        spb.weightgen(8, 0.1, 1, 1.0, 5.0, workdir=tmp_path)
+    for g in (4.0, -0.1):
+        with pytest.raises(spb.SpbteError, match=f"glance {g:g}"):
+            spb.weightgen(8, g, 1, 1.0, 5.0, workdir=tmp_path)
     exe = os.path.join(ROOT, "spectralbte_amd", "weightgen_amd")
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
