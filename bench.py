@@ -200,6 +200,7 @@ def main():
         extra["shock_run"] = guarded("shock run", shock_run, spb, a.form)
         extra["ensemble_mixed_run"] = guarded("mixed ensemble run", mixed_ensemble_run, spb, a.form)
         extra["landau_table"] = guarded("Landau table", landau_table, spb)
+        extra["glance_table"] = guarded("glancing-collision table", glance_table, spb)
     if not a.no_extra and not a.no_surrogate:
         extra["surrogate"] = guarded("surrogate", surrogate_run, spb, a.form)
 
@@ -267,6 +268,24 @@ def landau_table(spb, N=32, Lv=7.0, lam=1.0):
     log(f"Landau table N={N}: {sec:.2f} s, {n / sec:.3g} entries/s")
     return {"workload": f"Landau (glance 0) weight table, N={N}, L_v={Lv}, lambda={lam}, 1D grid", "seconds": round(sec, 2),
             "entries_per_s": round(n / sec, 1), "integrand_evals_per_s": round(n * 4096 / sec, 1)}
+
+
+def glance_table(spb, N=32, Lv=7.0, lam=1.0, glance=0.01):
+    """The anisotropic generator's glancing-collision (glance > 0) table on the
+    GPU, on the same grid: N^6 = 1.07e9 entries, each a 2-D sum over 128 theta
+    x 41 psi nodes of one tabulated radial function (the reference nests three
+    adaptive quadratures per entry)."""
+    from spectralbte_amd import workloads
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    t0 = time.perf_counter()
+    op.generate_weights_glance(lam, glance)
+    sec = time.perf_counter() - t0
+    op.close()
+    n = float(N) ** 6
+    log(f"glance table N={N}: {sec:.2f} s, {n / sec:.3g} entries/s")
+    return {"workload": f"glancing-collision weight table, N={N}, L_v={Lv}, lambda={lam}, glance={glance}, 1D grid",
+            "seconds": round(sec, 2), "entries_per_s": round(n / sec, 1)}
 
 
 def _mesh(nX, length, order):

@@ -77,8 +77,9 @@ int spbte_initial_state(const spbte_input *in, const species *sp, const double *
 int spbte_boltz_main(int argc, char **argv);
 
 /* exec/MPIWeightGenerator.c main on one GPU: argv = N glance beta lambda L_v.
- * glance 0 writes Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts (kept if it
- * exists); glance > 0 is refused (not built).  Returns 0 on success. */
+ * glance 0 writes Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts, 0 < glance < pi
+ * the glancing-collision table Weights/N%d_AnIso_L_v%g_lambda%g_glance%g.wts
+ * (exec/MPIWeightGenerator.c:80-83); an existing file is kept.  Returns 0 on success. */
 int spbte_weightgen_main(int argc, char **argv);
 
 #ifdef __cplusplus

@@ -26,6 +26,9 @@
  *   - orc_fft3d: numpy.fft with the explicit twiddles, and Gaussian closed form;
  *   - orc_qhat: the convolution theorem (W == 1) and single-column W;
  *   - orc_conserve: C.Q == 0, idempotence, numpy least-squares projection;
+ *   - orc_glance_entry: zero row mode, the xi = 0 and xi || zeta reductions
+ *     (scipy quadrature of the plane-wave identity), and the psi-average form
+ *     plus Lam/4 x orc_landau_entry for generic entries;
  *   - orc_upwind_two / orc_advect_two: parity unpinned by any reference fixture;
  *     pinned only by invariants (mass conservation, exactness on linear data,
  *     TVD on steps) in the tests.
