@@ -308,6 +308,30 @@ def test_reduced_precision_forms(spb, orc, N, Lv):
             assert np.abs(Cq).max() < 1e-13 * scale.max()
 
 
+def test_batch_beyond_one_pass_and_empty_batch(spb, orc):
+    """A batch larger than one pass (4096 cells at N=16: the cap of the
+    operator's per-pass scratch) runs in chunks: every cell, on both sides of
+    the chunk boundary, equals the same cell evaluated in a small batch with the
+    same tile, bit for bit; an empty batch is a no-op that touches no pointer."""
+    N, Lv = 16, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    op.set_tile("fp64", False, 4, 4, 1, 0)
+    rng = np.random.default_rng(4101)
+    ncell = 4096 + 5
+    V = np.stack(np.meshgrid(v, v, v, indexing="ij")).reshape(3, -1)
+    u = rng.uniform(-1, 1, (ncell, 3))
+    T = rng.uniform(0.6, 1.6, ncell)
+    d2 = ((V[None, :, :] - u[:, :, None]) ** 2).sum(1)
+    f = np.exp(-d2 / (2 * T[:, None])) / (2 * np.pi * T[:, None]) ** 1.5
+    Q = op.compute_q(f)
+    idx = [0, 1, 4095, 4096, 4097, ncell - 1]
+    assert np.array_equal(Q[idx], op.compute_q(f[idx]))
+    op.compute_q_dev(None, None, None, 0)
+    op.close()
+
+
 def test_n32_single_cell_against_oracle(spb, orc):
     """The headline grid (N=32, L_v=8, hard spheres, 1D grid): one cell, FP64 through
     the batched path against the oracle's full O(N^6) evaluation; FP32 forms <= 1e-6."""
