@@ -255,8 +255,9 @@ def shock_run(spb, form, steps=3):
 def landau_table(spb, N=32, Lv=7.0, lam=1.0):
     """The anisotropic generator's glance-0 (Landau) table on the GPU, on the
     1D grid of the heat-transfer configuration (N=32, L_v=7, lambda=1):
-    N^6 = 1.07e9 entries, each 64 x 64 Gauss-Legendre nodes (4096 J0 and
-    sincos evaluations)."""
+    N^6 = 1.07e9 entries, each 64 x 64 Gauss-Legendre nodes (the two theta
+    nodes of a symmetric pair give equal terms: 2048 J0 and sincos
+    evaluations)."""
     from spectralbte_amd import workloads
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
@@ -267,13 +268,13 @@ def landau_table(spb, N=32, Lv=7.0, lam=1.0):
     n = float(N) ** 6
     log(f"Landau table N={N}: {sec:.2f} s, {n / sec:.3g} entries/s")
     return {"workload": f"Landau (glance 0) weight table, N={N}, L_v={Lv}, lambda={lam}, 1D grid", "seconds": round(sec, 2),
-            "entries_per_s": round(n / sec, 1), "integrand_evals_per_s": round(n * 4096 / sec, 1)}
+            "entries_per_s": round(n / sec, 1), "gl_nodes_per_s": round(n * 4096 / sec, 1)}
 
 
 def glance_table(spb, N=32, Lv=7.0, lam=1.0, glance=0.01):
     """The anisotropic generator's glancing-collision (glance > 0) table on the
-    GPU, on the same grid: N^6 = 1.07e9 entries, each a 2-D sum over 128 theta
-    x 41 psi nodes of one tabulated radial function (the reference nests three
+    GPU, on the same grid: N^6 = 1.07e9 entries, each a 2-D sum over 112 theta
+    x 33 psi nodes of one tabulated radial function (the reference nests three
     adaptive quadratures per entry)."""
     from spectralbte_amd import workloads
     v, eta = workloads.grid_1d(N, Lv)

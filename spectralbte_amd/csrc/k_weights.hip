@@ -133,13 +133,16 @@ void gauss_legendre20(GL20 &q) {
 //   g = sin(t) J0(rho a0 sin t) (-rho^2 a1 sin^2 t cos(rho a2 cos t) + 4 rho a3 sin(rho a2 cos t) cos t),
 //   a0 = |xi_perp|, a1 = |zeta|^2, a2 = xi.zeta/|zeta|, a3 = |zeta| (zeta = row mode, xi = column mode),
 // both integrals by the 64-point Gauss-Legendre rule summed in symmetric node
-// pairs, as gsl_integration_glfixed and gauss_legendre do.  The node
-// abscissae, their sines and cosines and rho^(lam+2) are the same for every
-// entry and are evaluated once on the host with the C library.
+// pairs, as gsl_integration_glfixed and gauss_legendre do.  The theta nodes of
+// a pair, pi/2 + y and pi/2 - y with y = (pi/2) x_k, share sin(theta) = cos(y)
+// and have opposite cos(theta) = -+sin(y); g is even in cos(theta) (cos(rho a2
+// cos t) is even, sin(rho a2 cos t) cos t is even), so the two terms of a pair
+// are equal and each pair costs one J0 and one sincos.  The node abscissae,
+// cos(y), sin(y) and rho^(lam+2) are the same for every entry and are
+// evaluated once on the host with the C library.
 struct LandauRule {
   double w[32];                    // weights of the positive nodes, ascending nodes
-  double st_p[32], ct_p[32];       // sin, cos of theta = B + A x_k on [0, pi]
-  double st_m[32], ct_m[32];       // sin, cos of theta = B - A x_k
+  double st[32], ct[32];           // sin(theta), |cos(theta)| of theta = pi/2 -+ (pi/2) x_k
   double r_p[32], r_m[32];         // rho = B + A x_i, B - A x_i on [0, L]
   double rl_p[32], rl_m[32];       // rho^(lam+2)
   double At, Ar;                   // half-lengths of the two intervals
@@ -173,8 +176,7 @@ __global__ void __launch_bounds__(256) k_weights_landau(int N, const double *__r
       const double rho = h ? q.r_m[i] : q.r_p[i];
       double st = 0.0;
       for (int k = 0; k < 32; ++k)
-        st += q.w[k] * (landau_g(rho, a0, zetalen2, a2, zetalen, q.st_p[k], q.ct_p[k]) +
-                        landau_g(rho, a0, zetalen2, a2, zetalen, q.st_m[k], q.ct_m[k]));
+        st += q.w[k] * (2.0 * landau_g(rho, a0, zetalen2, a2, zetalen, q.st[k], q.ct[k]));
       f[h] = (h ? q.rl_m[i] : q.rl_p[i]) * (q.At * st);
     }
     sr += q.w[i] * (f[0] + f[1]);
@@ -215,13 +217,11 @@ extern "C" int spbte_launch_weights_landau(const spbte_grid_dev *g, const double
   gauss_legendre_half(64, x, q.w);
   q.At = 0.5 * (M_PI - 0.0);
   q.Ar = 0.5 * (g->Lv - 0.0);
-  const double Bt = 0.5 * (M_PI + 0.0), Br = 0.5 * (g->Lv + 0.0);
+  const double Br = 0.5 * (g->Lv + 0.0);
   for (int k = 0; k < 32; k++) {
     const double At_x = q.At * x[k], Ar_x = q.Ar * x[k];
-    q.st_p[k] = sin(Bt + At_x);
-    q.ct_p[k] = cos(Bt + At_x);
-    q.st_m[k] = sin(Bt - At_x);
-    q.ct_m[k] = cos(Bt - At_x);
+    q.st[k] = cos(At_x);
+    q.ct[k] = sin(At_x);
     q.r_p[k] = Br + Ar_x;
     q.r_m[k] = Br - Ar_x;
     q.rl_p[k] = pow(q.r_p[k], lam + 2);
@@ -261,22 +261,43 @@ extern "C" int spbte_launch_weights_landau(const spbte_grid_dev *g, const double
 //   R2 part = (pi Lam / 2) [2 |zeta|^2 (c_perp^2 E1(q) + (2 - c_perp^2) E2(q)) - 8 |zeta| c_par E3(q)],
 //   q = |xi|, c_perp = xi_perp / q, c_par = a1 / q,
 //   E1 = int r^(lam+4) S''(q r) dr,  E2 = int r^(lam+4) S'(q r)/(q r) dr,  E3 = int r^(lam+3) S'(q r) dr.
-// D, E1, E2, E3 are tabulated once per table on the host (piecewise Chebyshev,
-// degree 15, panels of width 3/L; values from composite Gauss-Legendre sums in
-// r), so an entry costs a 2-D sum of Chebyshev evaluations instead of a 3-D
-// sum of Bessel functions.  theta: 16-point panels in log(theta) on
+// D, E1, E2, E3 are tabulated once per table on the host (values from
+// composite Gauss-Legendre sums in r) on panels of width 3/L: D, evaluated at
+// every node, as degree-15 polynomials in the local variable t in [-1, 1]
+// (Chebyshev interpolants converted to monomials, evaluated by Horner's rule;
+// the coefficients stay of the size of the function, and the interpolation
+// error is about 1e-15 of it), held in LDS; E1..E3, evaluated once per entry,
+// as degree-15 Chebyshev series, read from global memory.  Narrower panels of
+// lower degree would read fewer bytes per node, but the lanes of a wave then
+// spread over more panels than the LDS banks separate.  An entry then costs a 2-D sum of polynomial evaluations instead of
+// a 3-D sum of Bessel functions.  theta: 16-point panels in log(theta) on
 // [sqrt(g), 1] and in theta on [max(1, sqrt(g)), pi]; psi: the trapezoid rule,
 // exact to rounding for the smooth even periodic integrand.
 namespace {
 
 struct GlanceDims {
   int nt, np;              // theta nodes, psi nodes
-  int nD, nE;              // Chebyshev panels of D and of E1..E3
+  int nD, nE;              // panels of D and of E1..E3
   double invhD, invhE;     // 1 / panel widths
   double lam4;             // pi Lam / 2
-  int total;               // doubles in the rule buffer
+  int nlds;                // doubles of the rule buffer copied to LDS (nodes and D)
 };
-constexpr int kChebN = 16, kChebStride = 17;  // coefficients per panel, padded stride (LDS banks)
+constexpr int kChebN = 16, kChebStride = 17;  // Chebyshev coefficients per panel, padded stride
+// monomial coefficients per panel; an odd stride in doubles puts 32 consecutive
+// panels on distinct LDS banks (the lanes of a wave read panels close together)
+constexpr int kPolyN = 16, kPolyStride = 17;
+
+__device__ __forceinline__ double poly_eval(const double *tab, int P, double invh, double x) {
+  const double u = x * invh;
+  int p = (int)u;
+  p = p < 0 ? 0 : (p >= P ? P - 1 : p);
+  const double t = 2.0 * (u - p) - 1.0;
+  const double *c = tab + p * kPolyStride;
+  double r = c[kPolyN - 1];
+#pragma unroll
+  for (int k = kPolyN - 2; k >= 0; --k) r = fma(r, t, c[k]);
+  return r;
+}
 
 __device__ __forceinline__ double cheb_eval(const double *tab, int P, double invh, double x) {
   const double u = x * invh;
@@ -296,18 +317,20 @@ __device__ __forceinline__ double cheb_eval(const double *tab, int P, double inv
 
 // rule buffer: s2[nt] = sin^2(theta/2), sth[nt] = sin(theta), wt[nt] = theta
 // weight x b(theta) x 8 pi^2; cps[np] = cos(psi), wps[np] = psi weight / pi;
-// D table (nD panels), E1, E2, E3 tables (nE panels each).
-__global__ void __launch_bounds__(256) k_weights_glance(int N, const double *__restrict__ eta, long r0, long nrows,
+// D table (nD panels): the first nlds doubles, copied to LDS; then the E1,
+// E2, E3 tables (nE panels each), read in place.
+constexpr int kGlanceThreads = 256;
+__global__ void __launch_bounds__(kGlanceThreads) k_weights_glance(int N, const double *__restrict__ eta, long r0, long nrows,
                                                         const double *__restrict__ rule, GlanceDims d,
                                                         double *__restrict__ W) {
   extern __shared__ double sm[];
-  for (int i = threadIdx.x; i < d.total; i += 256) sm[i] = rule[i];
+  for (int i = threadIdx.x; i < d.nlds; i += kGlanceThreads) sm[i] = rule[i];
   __syncthreads();
   const double *s2 = sm, *sth = s2 + d.nt, *wt = sth + d.nt, *cps = wt + d.nt, *wps = cps + d.np;
-  const double *tD = wps + d.np, *tE1 = tD + d.nD * kChebStride, *tE2 = tE1 + d.nE * kChebStride,
-               *tE3 = tE2 + d.nE * kChebStride;
+  const double *tD = wps + d.np;
+  const double *tE1 = rule + d.nlds, *tE2 = tE1 + d.nE * kChebStride, *tE3 = tE2 + d.nE * kChebStride;
   const long n3 = (long)N * N * N;
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  const long e = (long)blockIdx.x * kGlanceThreads + threadIdx.x;
   if (e >= nrows * n3) return;
   const long r = r0 + e / n3, c = e % n3;
   const double z1 = eta[r / ((long)N * N)], z2 = eta[(r / N) % N], z3 = eta[r % N];
@@ -322,14 +345,14 @@ __global__ void __launch_bounds__(256) k_weights_glance(int N, const double *__r
   const double a1 = zetalen != 0 ? xizeta / zetalen : 0.0;
   const double q2 = a1 * a1 + xp * xp, q = sqrt(q2);
   const double P = zetalen2 - 2.0 * zetalen * a1, Q = xp * zetalen;
-  const double D0 = cheb_eval(tD, d.nD, d.invhD, q);
+  const double D0 = poly_eval(tD, d.nD, d.invhD, q);
   double t1 = 0.0;
   for (int i = 0; i < d.nt; ++i) {
     const double A = fma(P, s2[i], q2), B = Q * sth[i];
     double acc = 0.0;
     for (int j = 0; j < d.np; ++j) {
       const double k2 = fma(-B, cps[j], A);
-      acc = fma(wps[j], cheb_eval(tD, d.nD, d.invhD, sqrt(k2 > 0.0 ? k2 : 0.0)), acc);
+      acc = fma(wps[j], poly_eval(tD, d.nD, d.invhD, sqrt(k2 > 0.0 ? k2 : 0.0)), acc);
     }
     t1 = fma(wt[i], acc - D0, t1);
   }
@@ -422,10 +445,33 @@ void cheb_table(int P, double h, F f, std::vector<double> &out) {
   }
 }
 
+// piecewise degree-(kPolyN-1) polynomials of f in t in [-1, 1] on [0, P h]:
+// Chebyshev interpolant at kPolyN nodes, converted to monomials in long double
+template <class F>
+void poly_table(int P, double h, F f, std::vector<double> &out) {
+  long double Tk[kPolyN][kPolyN] = {};  // monomial coefficients of T_k
+  Tk[0][0] = 1.0L;
+  Tk[1][1] = 1.0L;
+  for (int k = 2; k < kPolyN; ++k)
+    for (int i = 0; i < kPolyN; ++i) Tk[k][i] = (i > 0 ? 2.0L * Tk[k - 1][i - 1] : 0.0L) - Tk[k - 2][i];
+  double fv[kPolyN];
+  for (int p = 0; p < P; ++p) {
+    for (int j = 0; j < kPolyN; ++j) fv[j] = f(h * (p + 0.5 * (cos(M_PI * (j + 0.5) / kPolyN) + 1.0)));
+    long double a[kPolyN] = {};
+    for (int k = 0; k < kPolyN; ++k) {
+      long double c = 0.0L;
+      for (int j = 0; j < kPolyN; ++j) c += (long double)fv[j] * cosl(3.14159265358979323846264338327950288L * k * (j + 0.5L) / kPolyN);
+      c *= (k == 0 ? 1.0L : 2.0L) / kPolyN;
+      for (int i = 0; i < kPolyN; ++i) a[i] += c * Tk[k][i];
+    }
+    for (int i = 0; i < kPolyN; ++i) out.push_back((double)a[i]);
+    for (int i = kPolyN; i < kPolyStride; ++i) out.push_back(0.0);
+  }
+}
+
 }  // namespace
 
-// Limits of the glance table: 0 < g < pi (log sin(g/2) < 0); the rule and
-// tables must fit the workgroup's LDS.
+// The glance table: 0 < g < pi (log sin(g/2) < 0), checked by the caller.
 extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double *eta_host, const double *eta,
                                            double lam, double glance, double *W, hipStream_t s) {
   const int N = g->N;
@@ -452,9 +498,9 @@ extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double
   };
   const double t0 = sqrt(glance);
   if (t0 < 1.0) panels(log(t0), 0.0, 1 + (int)ceil(-log(t0) / 1.2), true);
-  if (t0 < M_PI) panels(fmax(t0, 1.0), M_PI, 2 + (int)ceil(L * zmax / 40.0), false);
+  if (t0 < M_PI) panels(fmax(t0, 1.0), M_PI, 2 + (int)ceil(L * zmax / 60.0), false);
   d.nt = (int)th.size();
-  const int M = 8 * (int)ceil((16.0 + L * zmax / 4.0) / 8.0);  // psi intervals
+  const int M = 8 * (int)ceil((12.0 + L * zmax / 5.0) / 8.0);  // psi intervals
   d.np = M + 1;
   for (int i = 0; i < d.nt; ++i) buf.push_back(sin(0.5 * th[i]) * sin(0.5 * th[i]));
   for (int i = 0; i < d.nt; ++i) buf.push_back(sin(th[i]));
@@ -462,32 +508,36 @@ extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double
     buf.push_back(8.0 * M_PI * M_PI * tw[i] * (8.0 / M_PI) * glance / (th[i] * th[i] * th[i] * th[i]));
   for (int j = 0; j <= M; ++j) buf.push_back(cos(M_PI * j / M));
   for (int j = 0; j <= M; ++j) buf.push_back((j == 0 || j == M ? 0.5 : 1.0) / M);
-  // radial tables
+  // radial tables: D in panels of width 3/L, widened if the LDS part would
+  // pass 64 KiB (very large N only; the error grows as the 16th power of the width)
+  double hp = 3.0 / L;
+  while ((buf.size() + (size_t)kPolyStride * ((int)ceil(kmax / hp) + 1)) * sizeof(double) > 64 * 1024) hp *= 1.1;
+  d.nD = (int)ceil(kmax / hp) + 1;
+  d.invhD = 1.0 / hp;
   const double hc = 3.0 / L;
-  d.nD = (int)ceil(kmax / hc) + 1;
   d.nE = (int)ceil(zmax / hc) + 1;
-  d.invhD = d.invhE = 1.0 / hc;
+  d.invhE = 1.0 / hc;
   {
-    const RadialRule r2(L, d.nD * hc, lam + 2.0), r3(L, d.nE * hc, lam + 3.0), r4(L, d.nE * hc, lam + 4.0);
-    cheb_table(d.nD, hc, [&](double k) { return r2.sum(kSm1, k); }, buf);
+    const RadialRule r2(L, d.nD * hp, lam + 2.0), r3(L, d.nE * hc, lam + 3.0), r4(L, d.nE * hc, lam + 4.0);
+    poly_table(d.nD, hp, [&](double k) { return r2.sum(kSm1, k); }, buf);
+    d.nlds = (int)buf.size();
     cheb_table(d.nE, hc, [&](double k) { return r4.sum(kS2, k); }, buf);
     cheb_table(d.nE, hc, [&](double k) { return r4.sum(kS1x, k); }, buf);
     cheb_table(d.nE, hc, [&](double k) { return r3.sum(kS1, k); }, buf);
   }
   d.lam4 = 0.5 * M_PI * log(glance) / log(sin(0.5 * glance));
-  d.total = (int)buf.size();
-  const size_t lds = sizeof(double) * buf.size();
-  if (lds > 64 * 1024) return (int)hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * d.nlds, bytes = sizeof(double) * buf.size();
   double *drule = NULL;
-  hipError_t e = hipMalloc((void **)&drule, lds);
+  hipError_t e = hipMalloc((void **)&drule, bytes);
   if (e != hipSuccess) return (int)e;
-  e = hipMemcpy(drule, buf.data(), lds, hipMemcpyHostToDevice);
+  e = hipMemcpy(drule, buf.data(), bytes, hipMemcpyHostToDevice);
   // rows per launch: about 4M entries
   const long rows = n3 >= (4L << 20) ? 1 : (4L << 20) / n3;
   for (long r0 = 0; e == hipSuccess && r0 < n3; r0 += rows) {
     const long nr = r0 + rows <= n3 ? rows : n3 - r0;
-    const unsigned blocks = (unsigned)((nr * n3 + 255) / 256);
-    hipLaunchKernelGGL(k_weights_glance, dim3(blocks), dim3(256), lds, s, N, eta, r0, nr, drule, d, W + r0 * n3);
+    const unsigned blocks = (unsigned)((nr * n3 + kGlanceThreads - 1) / kGlanceThreads);
+    hipLaunchKernelGGL(k_weights_glance, dim3(blocks), dim3(kGlanceThreads), lds, s, N, eta, r0, nr, drule, d,
+                       W + r0 * n3);
     e = hipGetLastError();
   }
   const hipError_t e2 = hipStreamSynchronize(s), e3 = hipFree(drule);
