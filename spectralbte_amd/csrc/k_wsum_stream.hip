@@ -26,6 +26,7 @@
 namespace {
 
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte unit of the permuted table
 
 template <int N, int TR, int WAVES>
 struct SGeom {
@@ -64,9 +65,9 @@ struct SGeom {
 
 // Weight j (0..3) of a 16-byte unit pair of the permuted table, as a double.
 template <int FORM>
-__device__ __forceinline__ double wget(const uint4 &u, const uint4 &u2, int j) {
+__device__ __forceinline__ double wget(const u32x4 &u, const u32x4 &u2, int j) {
   if constexpr (FORM == 0) {
-    const uint4 &q = j < 2 ? u : u2;
+    const u32x4 &q = j < 2 ? u : u2;
     return (j & 1) ? __builtin_bit_cast(double, make_uint2(q.z, q.w)) : __builtin_bit_cast(double, make_uint2(q.x, q.y));
   } else {
     const unsigned d = j == 0 ? u.x : (j == 1 ? u.y : (j == 2 ? u.z : u.w));
@@ -133,8 +134,8 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
     for (int t = 0; t < TR; ++t) acc[b][t] = make_double2(0.0, 0.0);
 
   // W chunk pointer: unit index ((wt*NC4 + c4)*TR + t)*UPW*64 + h*64 + lane
-  const uint4 *Wl = Wp + (wt * NC4 * TR * UPW) * 64 + lane;
-  auto wload = [&](int c4, uint4 (&u)[TR][UPW]) {
+  const u32x4 *Wl = (const u32x4 *)Wp + (wt * NC4 * TR * UPW) * 64 + lane;
+  auto wload = [&](int c4, u32x4 (&u)[TR][UPW]) {
 #pragma unroll
     for (int t = 0; t < TR; ++t)
 #pragma unroll
@@ -142,7 +143,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   };
   // Two static register sets alternate over the 4-column chunks of W (a body
   // holds an even number of chunks), each loaded four cz steps before use.
-  uint4 wA[TR][UPW], wB[TR][UPW];
+  u32x4 wA[TR][UPW], wB[TR][UPW];
   wload(cx0 * N * (N / 4), wA);
 
   for (int cx = cx0; cx < cx1; ++cx) {
@@ -187,9 +188,21 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       // One 4-step chunk at body offset jb (a compile-time constant after
       // unrolling): wc holds its weights, wn receives the next chunk's.  `fl`
       // points at the refill entry of body step 0: line + BS + (k0 - 1 - cz_body) mod N.
-      auto chunk = [&](uint4 (&wc)[TR][UPW], uint4 (&wn)[TR][UPW], const double2 *const (&fl)[NP][TB],
+      auto chunk = [&](u32x4 (&wc)[TR][UPW], u32x4 (&wn)[TR][UPW], const double2 *const (&fl)[NP][TB],
                        const double2 *gz, int jb, int c4n) __attribute__((always_inline)) {
-        if (c4n < NC4) wload(c4n, wn);
+        // The prefetch of the step after the last is clamped to the last chunk
+        // (loaded and not used) rather than skipped: a conditional load would
+        // make the compiler copy both register sets at every chunk.  The
+        // empty asm marks this chunk's weights as defined here, so their
+        // conversions (and the wait for their loads, issued four steps
+        // earlier) are not moved up next to the loads; the scheduling barrier
+        // keeps the next chunk's loads ahead of this chunk's arithmetic.
+#pragma unroll
+        for (int t = 0; t < TR; ++t)
+#pragma unroll
+          for (int h = 0; h < UPW; ++h) asm volatile("" : "+v"(wc[t][h]));
+        wload(c4n < NC4 ? c4n : NC4 - 1, wn);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
           const int j = jb + jj;  // position in the body and in the window period
