@@ -136,3 +136,35 @@ def test_initialize_weights_aniso_glance_file_and_missing_file(spb, orc, tmp_pat
     Wg = op.get_weights()
     op.close()
     assert np.array_equal(call(tmp_path / "b"), Wg)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_heat_transfer_run_with_glance_table(spb, orc, order):
+    """The heat-transfer configuration with the anisotropic glancing-collision
+    table (grazing parameter 0.1, lambda 1, L_v 7): a Maxwellian at rho 1,
+    u 0, T 1.5 with 1% seeded Gaussian density noise, diffuse walls at T 1 and
+    T 2, Kn 0.05; the GPU run (graph path) against the oracle's 1D driver
+    (exec/boltz.c:285-354) with the same table, which the tests above pin.  As
+    the reference writes the table (no trapezoid or h_eta^3 factors), Q is
+    about 1e4 times the isotropic table's, so the explicit step needs dt near
+    1e-6 at Kn 0.05."""
+    N, Lv, nX, Kn, dt, steps = 8, 7.0, 6, 0.05, 2e-6, 3
+    v, eta = orc.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_glance(1.0, 0.1)
+    W = op.get_weights()
+    x, dx = orc.uniform_mesh(nX, 1.0, order)
+    noise = 1.0 + 0.01 * np.random.default_rng(128).standard_normal(nX)
+    f0 = np.stack([orc.maxwellian_1d(v, noise[c], 0.0, 1.5).ravel() for c in range(nX)])
+    run = spb.Run.mesh(op, f0, x, dx, order=order, dt=dt, Kn=Kn, bc=spb.BC_DIFFUSE)
+    run.use_graph(True)
+    states = []
+    orc.run_1d(N, v, eta, W, f0, x, dx, dt, steps, order, Kn, spb.BC_DIFFUSE,
+               callback=lambda t, f: states.append(f[order:nX + order].copy()))
+    for t in range(steps):
+        run.step(1)
+        g = run.state()
+        assert np.abs(g - states[t]).max() < 1e-12 * np.abs(states[t]).max(), t
+    assert np.abs(states[-1] - f0).max() > 1e-4 * np.abs(f0).max()  # the collisions did act
+    run.close()
+    op.close()
