@@ -103,7 +103,7 @@ int spbte_op_generate_weights_landau(spbte_op *op, double lambda);
  * cross section (8/pi) glance / theta^4 on [sqrt(glance), pi] and the Taylor
  * term for [0, sqrt(glance)], at every (row, column) pair, on the GPU.  The
  * reference's nested adaptive rules (tolerance 1e-6) are replaced by a reduced
- * form of the same integral computed to about 1e-10.  0 < glance < pi, else
+ * form of the same integral, within about 1e-12 of max|W| of it.  0 < glance < pi, else
  * SPBTE_ERR_ARG. */
 int spbte_op_generate_weights_glance(spbte_op *op, double lambda, double glance);
 int spbte_op_get_weights(spbte_op *op, double *W);
