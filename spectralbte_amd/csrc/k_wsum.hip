@@ -490,3 +490,29 @@ extern "C" int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStr
   hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s, (const uint4 *)in, (uint4 *)out, bytes / 16);
   return (int)hipGetLastError();
 }
+
+// ---- deterministic fill of the autotuner's inputs: values spread over
+// [-1, 1) from a hash of the index.  The clock under a power-limited FP64 load
+// depends on the operands (zeros switch less), so tiles are timed on data
+// that loads the chip the way real transforms do.
+namespace {
+__device__ __forceinline__ unsigned fill_hash(size_t i) {
+  unsigned h = (unsigned)i * 0x9E3779B1u ^ (unsigned)(i >> 32) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  return h ^ (h >> 15);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_fill_pattern(T *__restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (T)((double)(fill_hash(i) >> 8) * (2.0 / 16777216.0) - 1.0);
+}
+}  // namespace
+
+extern "C" int spbte_launch_fill_pattern(void *p, size_t n, int single, hipStream_t s) {
+  if (single) hipLaunchKernelGGL(k_fill_pattern<float>, dim3(4096), dim3(256), 0, s, (float *)p, n);
+  else hipLaunchKernelGGL(k_fill_pattern<double>, dim3(4096), dim3(256), 0, s, (double *)p, n);
+  return (int)hipGetLastError();
+}

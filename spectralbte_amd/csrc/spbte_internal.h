@@ -70,6 +70,8 @@ int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbt
                              hipStream_t s);
 int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s);
 int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
+/* n values of a fixed pseudo-random pattern in [-1, 1): float if single, else double */
+int spbte_launch_fill_pattern(void *p, size_t n, int single, hipStream_t s);
 int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
 int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb);
 int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
