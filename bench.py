@@ -274,9 +274,10 @@ def landau_table(spb, N=32, Lv=7.0, lam=1.0):
 
 def glance_table(spb, N=32, Lv=7.0, lam=1.0, glance=0.01):
     """The anisotropic generator's glancing-collision (glance > 0) table on the
-    GPU, on the same grid: N^6 = 1.07e9 entries, each a 2-D sum over 112 theta
-    x 33 psi nodes of one tabulated radial function (the reference nests three
-    adaptive quadratures per entry)."""
+    GPU, on the same grid: N^6 = 1.07e9 entries, each a 2-D sum over at most
+    112 theta x 33 psi nodes (fewer for rows of smaller |zeta|) of one
+    tabulated radial function (the reference nests three adaptive quadratures
+    per entry)."""
     from spectralbte_amd import workloads
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)

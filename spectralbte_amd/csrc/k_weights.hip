@@ -276,7 +276,10 @@ extern "C" int spbte_launch_weights_landau(const spbte_grid_dev *g, const double
 namespace {
 
 struct GlanceDims {
-  int nt, np;              // theta nodes, psi nodes
+  int nl;                  // theta nodes of the log(theta) part (16 per panel; 0 if glance >= 1)
+  int nvt, nvp;            // theta rule variants (2 + v uniform panels), psi variants (8 k intervals)
+  double tL, pL;           // per-row picks: v = ceil(|zeta| tL), k = ceil((12 + |zeta| pL) / 8)
+  int dofs;                // offset of the D table in LDS
   int nD, nE;              // panels of D and of E1..E3
   double invhD, invhE;     // 1 / panel widths
   double lam4;             // pi Lam / 2
@@ -315,10 +318,21 @@ __device__ __forceinline__ double cheb_eval(const double *tab, int P, double inv
   return fma(t, b1, c[0] - b2);
 }
 
-// rule buffer: s2[nt] = sin^2(theta/2), sth[nt] = sin(theta), wt[nt] = theta
-// weight x b(theta) x 8 pi^2; cps[np] = cos(psi), wps[np] = psi weight / pi;
-// D table (nD panels): the first nlds doubles, copied to LDS; then the E1,
-// E2, E3 tables (nE panels each), read in place.
+// Node counts per row.  Over the theta and psi sums kappa stays within
+// |xi| -+ |zeta| (|k - xi| <= |zeta| sin(theta/2)), so the number of
+// oscillations of D(kappa) there is bounded by L |zeta| / pi: a row takes the
+// theta rule with 2 + ceil(L |zeta| / 60) uniform panels and the psi rule with
+// 8 ceil((12 + L |zeta| / 5) / 8) intervals.  Every variant is built on the
+// host; variant v of the theta rule holds nl + 16 (2 + v) nodes, variant k of
+// the psi rule 8 k + 1.
+__host__ __device__ inline int glance_tofs(int nl, int v) { return 3 * (v * nl + 16 * (2 * v + v * (v - 1) / 2)); }
+__host__ __device__ inline int glance_pofs(int k) { return 8 * k * (k - 1) + 2 * (k - 1); }
+
+// rule buffer: theta variants v = 0 .. nvt-1, each s2[nt] = sin^2(theta/2),
+// sth[nt] = sin(theta), wt[nt] = theta weight x b(theta) x 8 pi^2; psi variants
+// k = 1 .. nvp, each cps[8k+1] = cos(psi), wps[8k+1] = psi weight / pi; the D
+// table (nD panels) at dofs: the first nlds doubles, copied to LDS; then the
+// E1, E2, E3 tables (nE panels each), read in place.
 constexpr int kGlanceThreads = 256;
 __global__ void __launch_bounds__(kGlanceThreads) k_weights_glance(int N, const double *__restrict__ eta, long r0, long nrows,
                                                         const double *__restrict__ rule, GlanceDims d,
@@ -326,8 +340,7 @@ __global__ void __launch_bounds__(kGlanceThreads) k_weights_glance(int N, const 
   extern __shared__ double sm[];
   for (int i = threadIdx.x; i < d.nlds; i += kGlanceThreads) sm[i] = rule[i];
   __syncthreads();
-  const double *s2 = sm, *sth = s2 + d.nt, *wt = sth + d.nt, *cps = wt + d.nt, *wps = cps + d.np;
-  const double *tD = wps + d.np;
+  const double *tD = sm + d.dofs;
   const double *tE1 = rule + d.nlds, *tE2 = tE1 + d.nE * kChebStride, *tE3 = tE2 + d.nE * kChebStride;
   const long n3 = (long)N * N * N;
   const long e = (long)blockIdx.x * kGlanceThreads + threadIdx.x;
@@ -345,12 +358,18 @@ __global__ void __launch_bounds__(kGlanceThreads) k_weights_glance(int N, const 
   const double a1 = zetalen != 0 ? xizeta / zetalen : 0.0;
   const double q2 = a1 * a1 + xp * xp, q = sqrt(q2);
   const double P = zetalen2 - 2.0 * zetalen * a1, Q = xp * zetalen;
+  int v = (int)ceil(zetalen * d.tL), k = (int)ceil((12.0 + zetalen * d.pL) * 0.125);
+  v = v < 0 ? 0 : (v >= d.nvt ? d.nvt - 1 : v);
+  k = k < 1 ? 1 : (k > d.nvp ? d.nvp : k);
+  const int nt = d.nl + 16 * (2 + v), np = 8 * k + 1;
+  const double *s2 = sm + glance_tofs(d.nl, v), *sth = s2 + nt, *wt = sth + nt;
+  const double *cps = sm + glance_tofs(d.nl, d.nvt) + glance_pofs(k), *wps = cps + np;
   const double D0 = poly_eval(tD, d.nD, d.invhD, q);
   double t1 = 0.0;
-  for (int i = 0; i < d.nt; ++i) {
+  for (int i = 0; i < nt; ++i) {
     const double A = fma(P, s2[i], q2), B = Q * sth[i];
     double acc = 0.0;
-    for (int j = 0; j < d.np; ++j) {
+    for (int j = 0; j < np; ++j) {
       const double k2 = fma(-B, cps[j], A);
       acc = fma(wps[j], poly_eval(tD, d.nD, d.invhD, sqrt(k2 > 0.0 ? k2 : 0.0)), acc);
     }
@@ -482,11 +501,10 @@ extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double
   const double zmax = sqrt(3.0) * emax, kmax = 2.0 * zmax;  // |xi|, |zeta| <= zmax; kappa <= |xi| + |zeta|
   std::vector<double> buf;
   GlanceDims d;
-  // theta rule
+  // theta and psi rules, every per-row variant
   double gx[8], gw[8];
   gauss_legendre_half(16, gx, gw);
-  std::vector<double> th, tw;
-  auto panels = [&](double a, double b, int P, bool logv) {
+  auto panels = [&](std::vector<double> &th, std::vector<double> &tw, double a, double b, int P, bool logv) {
     const double h = (b - a) / P;
     for (int p = 0; p < P; ++p)
       for (int i = 0; i < 8; ++i)
@@ -497,17 +515,28 @@ extern "C" int spbte_launch_weights_glance(const spbte_grid_dev *g, const double
         }
   };
   const double t0 = sqrt(glance);
-  if (t0 < 1.0) panels(log(t0), 0.0, 1 + (int)ceil(-log(t0) / 1.2), true);
-  if (t0 < M_PI) panels(fmax(t0, 1.0), M_PI, 2 + (int)ceil(L * zmax / 60.0), false);
-  d.nt = (int)th.size();
-  const int M = 8 * (int)ceil((12.0 + L * zmax / 5.0) / 8.0);  // psi intervals
-  d.np = M + 1;
-  for (int i = 0; i < d.nt; ++i) buf.push_back(sin(0.5 * th[i]) * sin(0.5 * th[i]));
-  for (int i = 0; i < d.nt; ++i) buf.push_back(sin(th[i]));
-  for (int i = 0; i < d.nt; ++i)
-    buf.push_back(8.0 * M_PI * M_PI * tw[i] * (8.0 / M_PI) * glance / (th[i] * th[i] * th[i] * th[i]));
-  for (int j = 0; j <= M; ++j) buf.push_back(cos(M_PI * j / M));
-  for (int j = 0; j <= M; ++j) buf.push_back((j == 0 || j == M ? 0.5 : 1.0) / M);
+  const int P1 = t0 < 1.0 ? 1 + (int)ceil(-log(t0) / 1.2) : 0;
+  d.nl = 16 * P1;
+  d.tL = L / 60.0;
+  d.pL = L / 5.0;
+  d.nvt = (int)ceil(zmax * d.tL) + 1;
+  d.nvp = (int)ceil((12.0 + zmax * d.pL) * 0.125);
+  for (int v = 0; v < d.nvt; ++v) {
+    std::vector<double> th, tw;
+    if (P1) panels(th, tw, log(t0), 0.0, P1, true);
+    panels(th, tw, fmax(t0, 1.0), M_PI, 2 + v, false);
+    for (size_t i = 0; i < th.size(); ++i) buf.push_back(sin(0.5 * th[i]) * sin(0.5 * th[i]));
+    for (size_t i = 0; i < th.size(); ++i) buf.push_back(sin(th[i]));
+    for (size_t i = 0; i < th.size(); ++i)
+      buf.push_back(8.0 * M_PI * M_PI * tw[i] * (8.0 / M_PI) * glance / (th[i] * th[i] * th[i] * th[i]));
+  }
+  for (int k = 1; k <= d.nvp; ++k) {
+    const int M = 8 * k;
+    for (int j = 0; j <= M; ++j) buf.push_back(cos(M_PI * j / M));
+    for (int j = 0; j <= M; ++j) buf.push_back((j == 0 || j == M ? 0.5 : 1.0) / M);
+  }
+  if (buf.size() != (size_t)(glance_tofs(d.nl, d.nvt) + glance_pofs(d.nvp + 1))) return (int)hipErrorInvalidValue;
+  d.dofs = (int)buf.size();
   // radial tables: D in panels of width 3/L, widened if the LDS part would
   // pass 64 KiB (very large N only; the error grows as the 16th power of the width)
   double hp = 3.0 / L;
