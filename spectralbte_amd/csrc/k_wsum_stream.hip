@@ -9,8 +9,9 @@
 //  * a lane owns TR consecutive output rows of one z-line and runs the whole
 //    sum over c itself, so there is no cross-lane reduction;
 //  * every lane of the workgroup handles the same summed mode c at the same
-//    time, so G_b[c] is wave-uniform: one broadcast LDS read per step and cell
-//    (the G x-planes are staged in LDS with the F planes);
+//    time, so G_b[c] is wave-uniform: it is read straight from global memory
+//    with scalar loads (no LDS traffic, no vector registers; staging the G
+//    x-planes in LDS held 32 KiB per workgroup and twice the LDS reads);
 //  * along cz the shifted operand F_b[sy][rz + N/2 - cz] of a lane's TR rows is a
 //    window sliding by one element per cz step: it lives in registers and takes
 //    one LDS load per step, shared by TR rows (the F x-planes of the cells sit in
@@ -94,7 +95,6 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 
   constexpr int LW = Gm::LW;
   __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
-  __shared__ __attribute__((aligned(16))) double2 sG[NP * TB * N2];          // [p][b][cy][cz]
 
   // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
   // dispatched round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th
@@ -160,10 +160,6 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       const int z = m < BS ? m - BS + N : m - BS;
       sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
     }
-    for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
-      const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
-      sG[e] = Gp[p][(size_t)cells[b] * n3 + (size_t)cx * N2 + yz];
-    }
     __syncthreads();
 
     for (int cy = 0; cy < N; ++cy) {
@@ -184,12 +180,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
           for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][Gm::pos(BS + ((k0 + t) % N))];
       const int c4base = (cx * N + cy) * (N / 4);
-      const double2 *gline = sG + (size_t)cy * N;  // G_b[cx][cy][:] of product p, cell b at + (p*TB+b)*N2
       // One 4-step chunk at body offset jb (a compile-time constant after
       // unrolling): wc holds its weights, wn receives the next chunk's.  `fl`
       // points at the refill entry of body step 0: line + BS + (k0 - 1 - cz_body) mod N.
       auto chunk = [&](u32x4 (&wc)[TR][UPW], u32x4 (&wn)[TR][UPW], const double2 *const (&fl)[NP][TB],
-                       const double2 *gz, int jb, int c4n) __attribute__((always_inline)) {
+                       const double2 *const (&gz)[NP][TB], int jb, int c4n) __attribute__((always_inline)) {
         // The prefetch of the step after the last is clamped to the last chunk
         // (loaded and not used) rather than skipped: a conditional load would
         // make the compiler copy both register sets at every chunk.  The
@@ -215,12 +210,12 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
               // complex products as explicit FMA chains: 4 ops per product, 2 for w
-              const double2 g0 = gz[(size_t)(0 * TB + b) * N2 + j], f0 = buf[0][b][sl];
+              const double2 g0 = gz[0][b][j], f0 = buf[0][b][sl];
               double pr = g0.x * f0.x, pi = g0.x * f0.y;
               pr = fma(-g0.y, f0.y, pr);
               pi = fma(g0.y, f0.x, pi);
               if constexpr (NP == 2) {
-                const double2 g1 = gz[(size_t)(1 * TB + b) * N2 + j], f1 = buf[1][b][sl];
+                const double2 g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
                 pr = fma(g1.x, f1.x, pr);
                 pi = fma(g1.x, f1.y, pi);
                 pr = fma(-g1.y, f1.y, pr);
@@ -256,7 +251,13 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
         for (int p = 0; p < NP; ++p)
 #pragma unroll
           for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + (BS + base) / Gm::G : line[p][b] + BS + base;
-        const double2 *gz = gline + it * BS;
+        // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar loads)
+        const double2 *gz[NP][TB];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+          for (int b = 0; b < TB; ++b)
+            gz[p][b] = Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS;
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
           const int c4 = c4base + it * (BS / 4) + k;
