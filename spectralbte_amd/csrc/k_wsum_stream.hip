@@ -11,7 +11,9 @@
 //  * every lane of the workgroup handles the same summed mode c at the same
 //    time, so G_b[c] is wave-uniform: it is read straight from global memory
 //    with scalar loads (no LDS traffic, no vector registers; staging the G
-//    x-planes in LDS held 32 KiB per workgroup and twice the LDS reads);
+//    x-planes in LDS held 32 KiB per workgroup and twice the LDS reads), except
+//    with FP64 weights and one product, where the G lines passing through L2
+//    crowd out the 8-byte W stream and the x-plane is staged in LDS instead;
 //  * along cz the shifted operand F_b[sy][rz + N/2 - cz] of a lane's TR rows is a
 //    window sliding by one element per cz step: it lives in registers and takes
 //    one LDS load per step, shared by TR rows (the F x-planes of the cells sit in
@@ -95,6 +97,8 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 
   constexpr int LW = Gm::LW;
   __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
+  constexpr bool SG = FORM != 0 || NP == 2;  // G by scalar loads (else staged in LDS)
+  __shared__ __attribute__((aligned(16))) double2 sG[SG ? 1 : NP * TB * N2];  // [p][b][cy][cz]
 
   // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
   // dispatched round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th
@@ -160,6 +164,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       const int z = m < BS ? m - BS + N : m - BS;
       sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
     }
+    if constexpr (!SG)
+      for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
+        const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
+        sG[e] = Gp[p][(size_t)cells[b] * n3 + (size_t)cx * N2 + yz];
+      }
     __syncthreads();
 
     for (int cy = 0; cy < N; ++cy) {
@@ -251,13 +260,15 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
         for (int p = 0; p < NP; ++p)
 #pragma unroll
           for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + (BS + base) / Gm::G : line[p][b] + BS + base;
-        // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar loads)
+        // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar
+        // loads), or the staged x-plane in LDS
         const double2 *gz[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
           for (int b = 0; b < TB; ++b)
-            gz[p][b] = Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS;
+            gz[p][b] = SG ? Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS
+                          : sG + (size_t)(p * TB + b) * N2 + (size_t)cy * N + it * BS;
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
           const int c4 = c4base + it * (BS / 4) + k;
