@@ -83,7 +83,12 @@ __device__ __forceinline__ double wget(const u32x4 &u, const u32x4 &u2, int j) {
   }
 }
 
-template <int N, int TR, int WAVES, int TB, int NP, int FORM>
+// NTW: W read with non-temporal loads.  The launcher sets it when the batch is
+// one cell group, so that every W element is read by one workgroup only: a
+// streaming read then runs about 11% faster (an 8 GiB read measured 6.88 TB/s
+// non-temporal against 6.18 TB/s), while batches keep W in L2 for the cell
+// groups that share an r-tile.
+template <int N, int TR, int WAVES, int TB, int NP, int FORM, bool NTW>
 __global__ void __launch_bounds__(64 * WAVES)
 k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
               const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
@@ -143,7 +148,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
     for (int t = 0; t < TR; ++t)
 #pragma unroll
-      for (int h = 0; h < UPW; ++h) u[t][h] = Wl[(((size_t)c4 * TR + t) * UPW + h) * 64];
+      for (int h = 0; h < UPW; ++h) {
+        const u32x4 *q = Wl + (((size_t)c4 * TR + t) * UPW + h) * 64;
+        if constexpr (NTW) u[t][h] = __builtin_nontemporal_load(q);
+        else u[t][h] = *q;
+      }
   };
   // Two static register sets alternate over the 4-column chunks of W (a body
   // holds an even number of chunks), each loaded four cz steps before use.
@@ -318,10 +327,16 @@ int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, in
   const int ncg = (a->ncell + TB - 1) / TB;
   const int nrt = (int)((size_t)N * N * N / Gm::RWG);
   const int per = (nrt * csplit * ncg + 7) / 8;  // workgroups per XCD slice
-  hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM>), dim3(8 * per), dim3(64 * WAVES), 0, s,
-                     (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
-                     (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
-                     (double2 *)Qp);
+  if (ncg == 1)  // every W element read once: non-temporal loads
+    hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, true>), dim3(8 * per), dim3(64 * WAVES), 0, s,
+                       (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
+                       (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
+                       (double2 *)Qp);
+  else
+    hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, false>), dim3(8 * per), dim3(64 * WAVES), 0, s,
+                       (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
+                       (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
+                       (double2 *)Qp);
   if (csplit > 1) {
     const size_t n = (size_t)a->ncell * N * N * N;
     hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384)), dim3(256), 0,
