@@ -101,6 +101,7 @@ static void die(const char *what) {
 }
 
 void generate_conv_weights_AnIso(double **conv_weights) {
+  spbte_abi_forget_weights(conv_weights);
   double *v = malloc(sizeof(double) * N);
   if (!v) die("generate_conv_weights_AnIso");
   for (int i = 0; i < N; i++) v[i] = -L_v + i * (2 * L_v / (N - 1));
@@ -136,7 +137,13 @@ void initialize_weights_AnIso(int nodes, double *zeta, double Lv, double lam, in
   lambda_ = lam;
   gl64();
   const size_t n3 = (size_t)N * N * N;
-  for (size_t i = 0; i < n3; i++) conv_weights[i] = malloc(n3 * sizeof(double));
+  /* The rows are (re)filled below: drop any device copy made from them. */
+  spbte_abi_forget_weights(conv_weights);
+  for (size_t i = 0; i < n3; i++)
+    if (!(conv_weights[i] = malloc(n3 * sizeof(double)))) {
+      fprintf(stderr, "spectralbte_amd: out of host memory allocating weight rows\n");
+      exit(1);
+    }
   if (glance == 0)
     snprintf(name, sizeof name, "Weights/N%d_Aniso_L_v%g_lambda%g_Landau.wts", N, L_v, lambda_);
   else

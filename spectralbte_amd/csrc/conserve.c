@@ -14,8 +14,6 @@
 #include "spbte.h"
 #include "spbte_internal.h"
 
-void spbte_abi_set_mass(double mass); /* collisions.c */
-
 static int N = 0, Ns = 0;
 static double dv;
 static double *v;       /* borrowed */

@@ -153,6 +153,12 @@ int spbte_op_n(struct spbte_op *op);
 int spbte_op_table_slot(struct spbte_op *op);
 int spbte_fail(int code, const char *fmt, ...);
 
+/* collisions.c: the collisions.h module's device table cache.  Forget drops
+ * the copy made from a conv_weights pointer (its rows were freed or
+ * rewritten); set_mass passes the species mass that conserve.h receives. */
+void spbte_abi_forget_weights(double **conv_weights);
+void spbte_abi_set_mass(double mass);
+
 #ifdef __cplusplus
 }
 #endif

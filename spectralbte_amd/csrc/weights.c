@@ -16,8 +16,6 @@
 #include "spbte.h"
 #include "spbte_internal.h"
 
-void spbte_abi_forget_weights(double **conv_weights); /* collisions.c */
-
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
 #endif
@@ -26,9 +24,18 @@ static int N;
 static double L_v, lambda_, deta, prefactor, diam_i, diam_j, mass_i, mass_j;
 static double *zeta; /* borrowed */
 
+static void *xmalloc(size_t bytes) {
+  void *p = malloc(bytes);
+  if (!p) {
+    fprintf(stderr, "spectralbte_amd: out of host memory allocating %zu bytes of weights\n", bytes);
+    exit(1);
+  }
+  return p;
+}
+
 void alloc_weights(int n, double ****conv_weights, int total_species) {
-  *conv_weights = malloc(total_species * sizeof(double **));
-  for (int i = 0; i < total_species; i++) (*conv_weights)[i] = malloc((size_t)n * n * n * sizeof(double *));
+  *conv_weights = xmalloc(total_species * sizeof(double **));
+  for (int i = 0; i < total_species; i++) (*conv_weights)[i] = xmalloc((size_t)n * n * n * sizeof(double *));
 }
 
 static void store(const char *name, double **conv_weights) {
@@ -62,7 +69,10 @@ void initialize_weights(int nodes, double *eta, double Lv, double lam, int weigh
   printf("%g %g %g %g %s %s \n", diam_i, diam_j, mass_i, mass_j, species_i.name, species_j.name);
 
   const size_t n3 = (size_t)N * N * N;
-  for (size_t i = 0; i < n3; i++) conv_weights[i] = malloc(n3 * sizeof(double));
+  /* The rows are (re)filled below: a device copy made from this pointer
+   * earlier by ComputeQ no longer matches them. */
+  spbte_abi_forget_weights(conv_weights);
+  for (size_t i = 0; i < n3; i++) conv_weights[i] = xmalloc(n3 * sizeof(double));
 
   if (!isoFlag) {
     if (strcmp(species_i.name, "default") == 0)
@@ -101,7 +111,8 @@ void initialize_weights(int nodes, double *eta, double Lv, double lam, int weigh
 }
 
 void generate_conv_weights_iso(double **conv_weights) {
-  double *v = malloc(sizeof(double) * N);
+  spbte_abi_forget_weights(conv_weights);
+  double *v = xmalloc(sizeof(double) * N);
   for (int i = 0; i < N; i++) v[i] = -L_v + i * (2 * L_v / (N - 1));
   spbte_op *op = NULL;
   if (spbte_op_create(&op, N, L_v, v, zeta) != SPBTE_OK ||

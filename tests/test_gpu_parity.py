@@ -166,6 +166,56 @@ def test_computeq_abi_bkw8(spb, orc):
     spb.dealloc_coll()
 
 
+def test_computeq_abi_sees_reinitialized_weights(spb, orc, tmp_path):
+    """ComputeQ keeps a device copy of the table keyed by the conv_weights
+    pointer; initialize_weights refilling the same pointer (src/weights.c:33-125)
+    must replace that copy, so the next ComputeQ uses the new rows."""
+    import ctypes
+    import shutil
+    N, Lv = 8, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    (tmp_path / "Weights").mkdir()
+    shutil.copy(os.path.join(GOLDEN, "BKW8", "N8_isotropic_L_v5_lambda0.wts"), tmp_path / "Weights")
+    L = spb.lib()
+    L.initialize_weights.restype = None
+    L.initialize_weights.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, spb.Species, spb.Species]
+    cw = ctypes.POINTER(ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))()
+    L.alloc_weights(N, ctypes.byref(cw), 1)
+    eta_c = np.ascontiguousarray(eta)
+    dp = ctypes.POINTER(ctypes.c_double)
+    sp = spb.default_species()
+    f = np.ascontiguousarray(orc.bkw_initial(N, v).ravel())
+    spb.initialize_coll(N, Lv, v, eta)
+
+    def compute_q():
+        Q = np.zeros_like(f)
+        L.ComputeQ(f.ctypes.data_as(dp), f.ctypes.data_as(dp), Q.ctypes.data_as(dp), cw[0])
+        return Q
+
+    def table():
+        return np.stack([np.ctypeslib.as_array(cw[0][r], shape=(N ** 3,)).copy() for r in range(N ** 3)])
+
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        L.initialize_weights(N, eta_c.ctypes.data_as(dp), Lv, 0.0, 0, 0, ctypes.cast(cw[0], ctypes.c_void_p), sp, sp)
+        W0 = table()
+        Q0 = compute_q()
+        assert rel_l2(Q0, orc.compute_q(N, v, eta, W0, f, f)) < 1e-12
+        # regenerate lambda = 1 into the same pointer (weightFlag 1: fresh table)
+        L.initialize_weights(N, eta_c.ctypes.data_as(dp), Lv, 1.0, 1, 0, ctypes.cast(cw[0], ctypes.c_void_p), sp, sp)
+    finally:
+        os.chdir(cwd)
+    W1 = table()
+    assert rel_l2(W1, W0) > 1e-3
+    Q1 = compute_q()
+    assert rel_l2(Q1, orc.compute_q(N, v, eta, W1, f, f)) < 1e-12
+    assert rel_l2(Q1, Q0) > 1e-3
+    L.dealloc_weights(N, cw[0])
+    spb.dealloc_coll()
+
+
 def test_conserve_abi(spb, orc):
     """conserveAllMoments (src/conserve.h:16), one and two species."""
     N, Lv = 8, 9.0
