@@ -792,6 +792,7 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
 int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags) {
   if (!op || ncell < 0 || form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ncell == 0) return SPBTE_OK;
+  if (!d_f || !d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   const int ch = chunk_cells(op);
   int rc = ensure_scratch(op, ncell < ch ? ncell : ch);
@@ -807,6 +808,8 @@ int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *
 }
 
 int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
+  if (!op || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ncell > 0 && !d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
   if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
   if (ncell <= 0) return SPBTE_OK;
   LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
@@ -830,13 +833,17 @@ int spbte_op_get_species(spbte_op *op, int *Ns, double *masses) {
 }
 
 int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups) {
-  if (ngroups <= 0) return SPBTE_OK;
+  if (!op || ngroups < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ngroups == 0) return SPBTE_OK;
+  if (!d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
   LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, op->s));
   return SPBTE_OK;
 }
 
 int spbte_transform(spbte_op *op, const double *d_in, double *d_out, int ncell, int invert) {
-  if (ncell <= 0) return SPBTE_OK;
+  if (!op || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ncell == 0) return SPBTE_OK;
+  if (!d_in || !d_out) return fail(SPBTE_ERR_ARG, "null buffer");
   const int ch = chunk_cells(op);
   int rc = ensure_scratch(op, ncell < ch ? ncell : ch);
   if (rc) return rc;
@@ -849,8 +856,11 @@ int spbte_transform(spbte_op *op, const double *d_in, double *d_out, int ncell, 
 }
 
 int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double *d_Qhat, int ncell, int form) {
-  if (ncell <= 0) return SPBTE_OK;
+  if (!op || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad form %d", form);
+  if (ncell == 0) return SPBTE_OK;
+  if (!d_Fhat || !d_Ghat || !d_Qhat) return fail(SPBTE_ERR_ARG, "null buffer");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   int rc = ensure_scratch(op, ncell);
   if (rc) return rc;
   const void *F = d_Fhat, *G = d_Ghat;

@@ -128,6 +128,20 @@ def test_run_and_net_entry_points_reject_bad_arguments(spb):
     assert "bad" in L.spbte_last_error().decode() or "null" in L.spbte_last_error().decode()
 
 
+def test_operator_entry_points_reject_null_operator(spb):
+    """The operator's compute, transform and projection entry points return
+    SPBTE_ERR_ARG for a null operator or negative counts instead of touching it."""
+    L = spb.lib()
+    assert L.spbte_compute_q(None, None, None, None, 1, 0, 0) == -1
+    assert L.spbte_qhat(None, None, None, None, 1, 0) == -1
+    assert L.spbte_transform(None, None, None, 1, 0) == -1
+    assert L.spbte_conserve(None, None, 1) == -1
+    assert L.spbte_conserve_pairs(None, None, 1) == -1
+    assert L.spbte_conserve_species(None, None, 1) == -1
+    assert L.spbte_compute_q(None, None, None, None, -1, 0, 0) == -1
+    assert "bad arguments" in L.spbte_last_error().decode()
+
+
 def test_output_flags_file(spb, tmp_path):
     """read_flags of src/output.c:417-443 on the reference's own flag files (all
     columns on), and a file that switches two columns off."""
