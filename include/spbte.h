@@ -149,7 +149,9 @@ int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
 
 /* Timing on the operator's stream with HIP events: runs spbte_compute_q iters
  * times and returns the mean wall time per call and the mean duration of the
- * weighted-sum kernel alone (both in milliseconds). */
+ * weighted-sum kernel alone (both in milliseconds).  The batch must fit one
+ * pass (spbte_compute_q's chunk) and iters is 1 to 100000; SPBTE_ERR_ARG
+ * otherwise. */
 int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
                          int iters, double *ms_per_call, double *ms_wsum);
 

@@ -241,6 +241,7 @@ int spbte_op_set_species(spbte_op *op, int Ns, const double *masses) {
 
 /* ----------------------------------------------------------- create/destroy */
 int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const double *eta) {
+  if (!out) return fail(SPBTE_ERR_ARG, "null output handle");
   *out = NULL;
   if (N < 2 || N > SPBTE_MAX_N || !v || !eta) return fail(SPBTE_ERR_ARG, "bad grid (N=%d)", N);
   int ndev = 0;
@@ -252,6 +253,10 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
   op->n3 = (size_t)N * N * N;
   op->v = malloc(sizeof(double) * N);
   op->eta = malloc(sizeof(double) * N);
+  if (!op->v || !op->eta) {
+    spbte_op_destroy(op);
+    return fail(SPBTE_ERR_ARG, "out of host memory");
+  }
   memcpy(op->v, v, sizeof(double) * N);
   memcpy(op->eta, eta, sizeof(double) * N);
   spbte_grid_dev *g = &op->g;
@@ -263,6 +268,12 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
   g->pre_fwd = scale3 * g->hv * g->hv * g->hv;
   g->pre_inv = scale3 * g->heta * g->heta * g->heta;
   double *Mf = malloc(sizeof(double) * 2 * N * N), *Mi = malloc(sizeof(double) * 2 * N * N), w[SPBTE_MAX_N];
+  if (!Mf || !Mi) {
+    free(Mf);
+    free(Mi);
+    spbte_op_destroy(op);
+    return fail(SPBTE_ERR_ARG, "out of host memory");
+  }
   const double Leta = -eta[0];
   build_matrix(N, eta, g->hv, Leta, L_v, 1.0, -1.0, Mf);
   build_matrix(N, v, g->heta, L_v, Leta, -1.0, 1.0, Mi);
@@ -881,38 +892,38 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
 /* ------------------------------------------------------------ timing/tuning */
 int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
                          int iters, double *ms_per_call, double *ms_wsum) {
-  if (iters < 1) return fail(SPBTE_ERR_ARG, "iters < 1");
+  if (!op || !d_f || !d_Q || !ms_per_call || !ms_wsum || ncell < 1 || form < 0 || form >= NFORMS)
+    return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (iters < 1 || iters > 100000) return fail(SPBTE_ERR_ARG, "iters must be 1 to 100000");
   const int ch = chunk_cells(op);
   if (ncell > ch) return fail(SPBTE_ERR_ARG, "timed batch larger than one chunk (%d cells)", ch);
   int rc = ensure_scratch(op, ncell);
   if (rc) return rc;
-  hipEvent_t e0, e1, *ka = calloc(iters, sizeof(hipEvent_t)), *kb = calloc(iters, sizeof(hipEvent_t));
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  for (int i = 0; i < iters; i++) {
-    HIPCHK(hipEventCreate(&ka[i]));
-    HIPCHK(hipEventCreate(&kb[i]));
-  }
-  HIPCHK(hipStreamSynchronize(op->s));
-  HIPCHK(hipEventRecord(e0, op->s));
-  for (int i = 0; i < iters && rc == SPBTE_OK; i++)
-    rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ka[i], kb[i]);
-  HIPCHK(hipEventRecord(e1, op->s));
-  HIPCHK(hipEventSynchronize(e1));
-  float tot = 0.f, k = 0.f, ksum = 0.f;
-  HIPCHK(hipEventElapsedTime(&tot, e0, e1));
-  for (int i = 0; i < iters; i++) {
-    if (rc == SPBTE_OK) {
-      HIPCHK(hipEventElapsedTime(&k, ka[i], kb[i]));
-      ksum += k;
+  /* events: [0] start, [1] end, then a begin/end pair around each weighted sum */
+  const int nev = 2 + 2 * iters;
+  hipEvent_t *ev = calloc((size_t)nev, sizeof(hipEvent_t));
+  if (!ev) return fail(SPBTE_ERR_ARG, "out of host memory");
+  int made = 0;
+  for (; made < nev; made++)
+    if (hipEventCreate(&ev[made]) != hipSuccess) {
+      rc = fail(SPBTE_ERR_HIP, "hipEventCreate failed");
+      break;
     }
-    hipEventDestroy(ka[i]);
-    hipEventDestroy(kb[i]);
+  float tot = 0.f, k = 0.f, ksum = 0.f;
+  if (!rc && (hipStreamSynchronize(op->s) != hipSuccess || hipEventRecord(ev[0], op->s) != hipSuccess))
+    rc = fail(SPBTE_ERR_HIP, "timing set-up failed");
+  for (int i = 0; i < iters && !rc; i++)
+    rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ev[2 + 2 * i], ev[3 + 2 * i]);
+  if (!rc && (hipEventRecord(ev[1], op->s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+              hipEventElapsedTime(&tot, ev[0], ev[1]) != hipSuccess))
+    rc = fail(SPBTE_ERR_HIP, "timing failed: %s", hipGetErrorString(hipGetLastError()));
+  for (int i = 0; i < iters && !rc; i++) {
+    if (hipEventElapsedTime(&k, ev[2 + 2 * i], ev[3 + 2 * i]) != hipSuccess)
+      rc = fail(SPBTE_ERR_HIP, "timing failed: %s", hipGetErrorString(hipGetLastError()));
+    ksum += k;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  free(ka);
-  free(kb);
+  for (int i = 0; i < made; i++) hipEventDestroy(ev[i]);
+  free(ev);
   if (rc) return rc;
   *ms_per_call = tot / iters;
   *ms_wsum = ksum / iters;

@@ -129,8 +129,10 @@ def test_run_and_net_entry_points_reject_bad_arguments(spb):
 
 
 def test_operator_entry_points_reject_null_operator(spb):
-    """The operator's compute, transform and projection entry points return
-    SPBTE_ERR_ARG for a null operator or negative counts instead of touching it."""
+    """The operator's compute, transform, projection and timing entry points
+    return SPBTE_ERR_ARG for a null operator or negative counts instead of
+    touching it."""
+    import ctypes
     L = spb.lib()
     assert L.spbte_compute_q(None, None, None, None, 1, 0, 0) == -1
     assert L.spbte_qhat(None, None, None, None, 1, 0) == -1
@@ -139,6 +141,8 @@ def test_operator_entry_points_reject_null_operator(spb):
     assert L.spbte_conserve_pairs(None, None, 1) == -1
     assert L.spbte_conserve_species(None, None, 1) == -1
     assert L.spbte_compute_q(None, None, None, None, -1, 0, 0) == -1
+    t = ctypes.c_double()
+    assert L.spbte_time_compute_q(None, None, None, None, 1, 0, 0, 3, ctypes.byref(t), ctypes.byref(t)) == -1
     assert "bad arguments" in L.spbte_last_error().decode()
 
 
