@@ -62,7 +62,7 @@ def test_fft3D_abi(spb, orc):
 
 
 # ------------------------------------------------------------- weighted sum
-@pytest.mark.parametrize("N,ncell", [(8, 2), (16, 3), (24, 3), (12, 2)])
+@pytest.mark.parametrize("N,ncell", [(8, 2), (16, 3), (24, 3), (12, 2), (9, 2), (10, 3)])
 def test_qhat_fp64_random(spb, orc, N, ncell):
     """Random W, F, G (ragged cell count vs the tile's cells per workgroup)."""
     v, eta = orc.grid_0d(N, 5.0)
@@ -75,6 +75,11 @@ def test_qhat_fp64_random(spb, orc, N, ncell):
     Q = op.qhat(F, G, "fp64")
     for b in range(ncell):
         assert rel_l2(Q[b], orc.qhat(N, W, F[b], G[b])) < 1e-12
+    if N in (9, 10, 12):
+        # grids without reduced-precision tiles run FP64 only, and say so
+        with pytest.raises(spb.SpbteError, match="reduced-precision"):
+            op.qhat(F, G, "fp32")
+    op.close()
 
 
 @pytest.mark.parametrize("N", [16, 24])
