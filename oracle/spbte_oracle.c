@@ -29,9 +29,11 @@
  *   - orc_glance_entry: zero row mode, the xi = 0 and xi || zeta reductions
  *     (scipy quadrature of the plane-wave identity), and the psi-average form
  *     plus Lam/4 x orc_landau_entry for generic entries;
- *   - orc_upwind_two / orc_advect_two: parity unpinned by any reference fixture;
- *     pinned only by invariants (mass conservation, exactness on linear data,
- *     TVD on steps) in the tests.
+ *   - orc_upwind_two / orc_advect_two: no reference fixture exists; pinned by
+ *     what the scheme must satisfy: exact translation of data linear in x on a
+ *     non-uniform mesh, total variation and range of step data not increased,
+ *     second-order convergence on smooth data, the body-force momentum
+ *     identity, and the one-wall case composed of the two-wall and copy cases.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -229,6 +229,68 @@ def test_transport_convergence_orders(orc):
     assert e2[-1] < 0.2 * e1[-1]
 
 
+def _nonuniform_mesh(nX, order, seed):
+    """Cell widths drawn in [0.05, 0.15], centres with faces that match:
+    x[l+1] - x[l] = (dx[l] + dx[l+1]) / 2 (ghosts included)."""
+    rng = np.random.default_rng(seed)
+    dx = rng.uniform(0.05, 0.15, nX + 2 * order)
+    x = np.concatenate([[0.0], np.cumsum(0.5 * (dx[:-1] + dx[1:]))])
+    return x, dx
+
+
+@pytest.mark.parametrize("stage", ["upwind_two", "advect_two"])
+def test_order2_transport_exact_on_linear_data(orc, stage):
+    """On data linear in x every minmod slope is the exact gradient (all three
+    divided differences of src/transportroutines.c:406-420,443-449 agree), so
+    the faces x_l +- dx_l/2 carry the exact values and one upwindTwo stage
+    translates the data by 0.5 dt v_i (CFL_NUM = 0.5 dt v/dx, :422,451) on any
+    mesh; advectTwo's two stages and average (:477-492) translate it by the
+    same amount.  A wrong width, face index, upwind side or time factor breaks
+    this on a non-uniform mesh.  Cells next to the walls and the extrapolated
+    ghosts (:271-274, :293-296) are excluded."""
+    N, Lv, nX = 4, 2.0, 16
+    v, _ = orc.grid_1d(N, Lv)
+    x, dx = _nonuniform_mesh(nX, 2, seed=3)
+    M = orc.maxwellian_1d(v, 1.0, 0.0, 1.0).ravel()
+    vx = np.repeat(v, N * N)
+    a, b = 1.3, -0.7
+    f = (a + b * x)[:, None] * M[None, :]
+    dt = 0.8 * dx.min() / Lv
+    if stage == "upwind_two":
+        _, fc = orc.upwind_two(N, nX, v, x, dx, dt, 0, f)
+        cells = range(4, nX)
+    else:
+        _, fc = orc.advect_two(N, nX, v, x, dx, dt, 0, f)
+        cells = range(6, nX - 2)
+    for l in cells:
+        exact = (a + b * (x[l] - 0.5 * dt * vx)) * M
+        assert np.abs(fc[l] - exact).max() <= 1e-13 * np.abs(f).max(), l
+
+
+def test_order2_transport_is_tvd_on_steps(orc):
+    """A minmod MUSCL stage with 0.5 dt |v|/dx <= 0.4 neither raises the total
+    variation of a velocity node's profile nor leaves the range of its data
+    (src/transportroutines.c:80-90,406-466): two steps on a non-uniform mesh,
+    away from the walls."""
+    N, Lv, nX = 4, 2.0, 24
+    v, _ = orc.grid_1d(N, Lv)
+    x, dx = _nonuniform_mesh(nX, 2, seed=5)
+    M = orc.maxwellian_1d(v, 1.0, 0.0, 1.0).ravel()
+    prof = np.where((np.arange(nX + 4) >= 9) & (np.arange(nX + 4) < 17), 1.0, 0.2)
+    prof[13:15] = 0.6
+    f = prof[:, None] * M[None, :]
+    dt = 0.8 * dx.min() / Lv
+    inner = slice(2, nX + 2)
+    for fn in (orc.upwind_two, orc.advect_two):
+        _, fc = fn(N, nX, v, x, dx, dt, 0, f)
+        tv0 = np.abs(np.diff(f[inner], axis=0)).sum(axis=0)
+        tv1 = np.abs(np.diff(fc[inner], axis=0)).sum(axis=0)
+        assert np.all(tv1 <= tv0 * (1 + 1e-12)), fn.__name__
+        assert np.all(fc[inner] <= f.max(axis=0) * (1 + 1e-12))
+        assert np.all(fc[inner] >= f.min(axis=0) * (1 - 1e-12))
+        assert np.abs(tv1 - tv0).max() > 0  # the data did move
+
+
 def test_left_diffuse_wall_composes_diffuse_and_copy(orc):
     """bc 3 (ICChoice 1: diffuse left wall, copied right ghost) equals the
     diffuse-wall result away from the right wall and the copy result away from
