@@ -9,9 +9,9 @@
  * crosses PCIe once.
  *
  * The precision form defaults to FP64 (the reference method); the environment
- * variable SPBTE_FORM = fp32 | fp16x2 | fp32a selects another.  HIP failures print a
- * message and exit(1), which is how the reference handles fatal errors
- * (src/weights.c:84-86).
+ * variable SPBTE_FORM = fp32 | fp16x2 | fp32a selects another (other values
+ * warn and keep FP64).  HIP failures print a message and exit(1), which is how
+ * the reference handles fatal errors (src/weights.c:84-86).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -45,6 +45,11 @@ static int form_from_env(void) {
   if (!strcmp(s, "fp32")) return SPBTE_FORM_FP32;
   if (!strcmp(s, "fp16x2")) return SPBTE_FORM_FP16X2;
   if (!strcmp(s, "fp32a")) return SPBTE_FORM_FP32A;
+  if (strcmp(s, "fp64")) {
+    static int warned;
+    if (!warned) fprintf(stderr, "spectralbte_amd: SPBTE_FORM=%s is not fp64, fp32, fp16x2 or fp32a; using fp64\n", s);
+    warned = 1;
+  }
   return SPBTE_FORM_FP64;
 }
 
