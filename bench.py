@@ -135,8 +135,8 @@ def flops_per_pair(mp):
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (cells would shard across "
-              "GPUs with a replicated weight table; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (cells would shard across "
+              "GPUs, each holding a replica of the weight table).", file=sys.stderr)
         sys.exit(2)
     import spectralbte_amd as spb
 
@@ -148,8 +148,9 @@ def main():
     ms, mk, tile, Q = wl.run(a.form, a.steps, a.warmup, tile)
     value = ncell / (ms * 1e-3)
 
-    # ---- roofline of the weighted-sum kernel: bound by FP64 VALU issue (it
-    # streams W through L2 shared by cell groups)
+    # ---- roofline of the weighted-sum kernel: bound by FP64 VALU issue (the
+    # cell groups of a batch share each streamed piece of W through L2, so HBM
+    # traffic per flop is small)
     flops = flops_per_pair(wl.mp) * P * ncell
     sw = {"fp64": 8, "fp32": 4, "fp16x2": 4, "fp32a": 4}[a.form]
     hbm_min = sw * P + ncell * n3 * 16 * (3 if not wl.mp else 5)   # W once + F/G read + Qhat written

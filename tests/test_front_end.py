@@ -183,7 +183,7 @@ def test_species_mass_enters_states_and_kb(spb):
 
 
 def test_boltz_main_errors_before_any_gpu_call(spb, tmp_path):
-    """The driver reports a missing input file (src/input.c:24-27) and refuses
+    """The front end reports a missing input file (src/input.c:24-27) and refuses
     runs it cannot do, before touching the GPU."""
     with pytest.raises(spb.SpbteError, match="input file not found"):
         spb.boltz("absent.in", "absent.out", workdir=tmp_path)

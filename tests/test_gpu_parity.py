@@ -1,6 +1,6 @@
 """GPU parity: every kernel of the MI355X operator against the CPU oracle on the
 same seeded inputs, through the C ABI.  Tolerances: FP64 paths 1e-12 relative
-L2 (the north-star parity bar; transforms 1e-13); reduced-precision forms 1e-6
+L2 (the reference-method bar; transforms 1e-13); reduced-precision forms 1e-6
 relative L2 against the FP64 result; conservation |C.Q| <= 1e-13 relative."""
 import os
 
