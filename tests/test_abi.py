@@ -214,3 +214,15 @@ def test_initialize_weights_aniso_loads_landau_file_and_ghat3l(spb, orc, tmp_pat
         assert abs(got - ref) <= 1e-14 * abs(ref)
         assert abs(got - W[(i * N + j) * N + k, (l * N + m) * N + n]) <= 1e-14 * abs(ref)
     L.dealloc_weights(N, rows)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 (or a multi-rank launch) exits
+    with a message and status 2 before touching a device."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for args, env in ((["--gpus", "2"], {}), ([], {"WORLD_SIZE": "4"})):
+        r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), *args], capture_output=True, text=True,
+                           timeout=120, env=dict(os.environ, **env))
+        assert r.returncode == 2
+        assert "one GPU" in r.stderr and r.stdout == ""
