@@ -66,9 +66,10 @@ struct SGeom {
   static_assert((N * N * N) % RWG == 0, "workgroups must tile the output rows");
 };
 
-// Weight j (0..3) of a 16-byte unit pair of the permuted table, as a double.
+// Weight j (0..3) of a 16-byte unit pair of the permuted table: a double, or
+// the FP32 value itself in the FP32-product form (FORM 3).
 template <int FORM>
-__device__ __forceinline__ double wget(const u32x4 &u, const u32x4 &u2, int j) {
+__device__ __forceinline__ auto wget(const u32x4 &u, const u32x4 &u2, int j) {
   if constexpr (FORM == 0) {
     const u32x4 &q = j < 2 ? u : u2;
     return (j & 1) ? __builtin_bit_cast(double, make_uint2(q.z, q.w)) : __builtin_bit_cast(double, make_uint2(q.x, q.y));
@@ -76,6 +77,8 @@ __device__ __forceinline__ double wget(const u32x4 &u, const u32x4 &u2, int j) {
     const unsigned d = j == 0 ? u.x : (j == 1 ? u.y : (j == 2 ? u.z : u.w));
     if constexpr (FORM == 1) {
       return (double)__builtin_bit_cast(float, d);
+    } else if constexpr (FORM == 3) {
+      return __builtin_bit_cast(float, d);
     } else {
       const h2v hl = __builtin_bit_cast(h2v, d);
       return (double)hl.x + (double)hl.y;
@@ -98,12 +101,15 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   constexpr size_t n3 = (size_t)N * N * N;
   constexpr int NRT = (int)(n3 / RWG);                  // r-tiles
   constexpr int UPW = (FORM == 0) ? 2 : 1;              // 16-byte units per 4 weights
+  // F, G elements: FP32 complex in the FP32-product form (products and a
+  // 4-step partial sum in FP32, folded into the FP64 accumulators), else FP64
+  using C = std::conditional_t<FORM == 3, float2, double2>;
   constexpr int BS = Gm::BS;                            // cz steps per loop body
 
   constexpr int LW = Gm::LW;
-  __shared__ __attribute__((aligned(16))) double2 sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
+  __shared__ __attribute__((aligned(16))) C sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
   constexpr bool SG = FORM != 0 || NP == 2;  // G by scalar loads (else staged in LDS)
-  __shared__ __attribute__((aligned(16))) double2 sG[SG ? 1 : NP * TB * N2];  // [p][b][cy][cz]
+  __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * TB * N2];  // [p][b][cy][cz]
 
   // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
   // dispatched round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th
@@ -127,8 +133,8 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   const int k0 = (rz0 + N / 2) % N;                      // window start at cz = 0 (non-negative)
   const size_t wt = (size_t)rt * WAVES + wave;           // wave tile of the permuted table
 
-  const double2 *Fp[2] = {F0, F1};
-  const double2 *Gp[2] = {G0, G1};
+  const C *Fp[2] = {(const C *)F0, (const C *)F1};
+  const C *Gp[2] = {(const C *)G0, (const C *)G1};
   int cells[TB];
 #pragma unroll
   for (int b = 0; b < TB; ++b) {
@@ -137,10 +143,15 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   }
 
   double2 acc[TB][TR];
+  float2 a32[FORM == 3 ? TB : 1][TR];  // FORM 3: FP32 partial sums of one 4-step chunk
 #pragma unroll
   for (int b = 0; b < TB; ++b)
 #pragma unroll
     for (int t = 0; t < TR; ++t) acc[b][t] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int b = 0; b < (FORM == 3 ? TB : 1); ++b)
+#pragma unroll
+    for (int t = 0; t < TR; ++t) a32[b][t] = make_float2(0.f, 0.f);
 
   // W chunk pointer: unit index ((wt*NC4 + c4)*TR + t)*UPW*64 + h*64 + lane
   const u32x4 *Wl = (const u32x4 *)Wp + (wt * NC4 * TR * UPW) * 64 + lane;
@@ -183,14 +194,14 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
     for (int cy = 0; cy < N; ++cy) {
       int sy = ry + N / 2 - cy;
       sy = sy < 0 ? sy + N : (sy > N - 1 ? sy - N : sy);
-      const double2 *line[NP][TB];
+      const C *line[NP][TB];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * TB + b) * P + pl) * N + sy) * LW;
       // circular window: at step cz, element t (row rz0 + t) is buf[(t - cz) mod TR]
       // and holds F_line[(k0 + t - cz) mod N] = line[BS + (k0 + t - cz) mod N]
-      double2 buf[NP][TB][TR];
+      C buf[NP][TB][TR];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -201,8 +212,8 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       // One 4-step chunk at body offset jb (a compile-time constant after
       // unrolling): wc holds its weights, wn receives the next chunk's.  `fl`
       // points at the refill entry of body step 0: line + BS + (k0 - 1 - cz_body) mod N.
-      auto chunk = [&](u32x4 (&wc)[TR][UPW], u32x4 (&wn)[TR][UPW], const double2 *const (&fl)[NP][TB],
-                       const double2 *const (&gz)[NP][TB], int jb, int c4n) __attribute__((always_inline)) {
+      auto chunk = [&](u32x4 (&wc)[TR][UPW], u32x4 (&wn)[TR][UPW], const C *const (&fl)[NP][TB],
+                       const C *const (&gz)[NP][TB], int jb, int c4n) __attribute__((always_inline)) {
         // The prefetch of the step after the last is clamped to the last chunk
         // (loaded and not used) rather than skipped: a conditional load would
         // make the compiler copy both register sets at every chunk.  The
@@ -224,23 +235,28 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
           for (int t = TR - 1; t >= 0; --t) {
             const int sl = ((t - j) % TR + TR) % TR;
-            const double w = wget<FORM>(wc[t][0], wc[t][UPW - 1], jj);
+            const auto w = wget<FORM>(wc[t][0], wc[t][UPW - 1], jj);
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
               // complex products as explicit FMA chains: 4 ops per product, 2 for w
-              const double2 g0 = gz[0][b][j], f0 = buf[0][b][sl];
-              double pr = g0.x * f0.x, pi = g0.x * f0.y;
+              const C g0 = gz[0][b][j], f0 = buf[0][b][sl];
+              auto pr = g0.x * f0.x, pi = g0.x * f0.y;
               pr = fma(-g0.y, f0.y, pr);
               pi = fma(g0.y, f0.x, pi);
               if constexpr (NP == 2) {
-                const double2 g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
+                const C g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
                 pr = fma(g1.x, f1.x, pr);
                 pi = fma(g1.x, f1.y, pi);
                 pr = fma(-g1.y, f1.y, pr);
                 pi = fma(g1.y, f1.x, pi);
               }
-              acc[b][t].x = fma(w, pr, acc[b][t].x);
-              acc[b][t].y = fma(w, pi, acc[b][t].y);
+              if constexpr (FORM == 3) {
+                a32[b][t].x = fmaf(w, pr, a32[b][t].x);
+                a32[b][t].y = fmaf(w, pi, a32[b][t].y);
+              } else {
+                acc[b][t].x = fma(w, pr, acc[b][t].x);
+                acc[b][t].y = fma(w, pi, acc[b][t].y);
+              }
             }
             if (t == TR - 1) {
               const int slnew = ((-1 - j) % TR + TR) % TR;
@@ -257,6 +273,17 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
             }
           }
         }
+        if constexpr (FORM == 3) {
+          // fold the chunk's FP32 partial sums (4 terms each) into FP64
+#pragma unroll
+          for (int b = 0; b < TB; ++b)
+#pragma unroll
+            for (int t = 0; t < TR; ++t) {
+              acc[b][t].x += (double)a32[b][t].x;
+              acc[b][t].y += (double)a32[b][t].y;
+              a32[b][t] = make_float2(0.f, 0.f);
+            }
+        }
       };
       // The body is a whole number of window periods (TR) and of chunk pairs,
       // so window slots and the wA/wB roles are the same at every iteration.
@@ -264,14 +291,14 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll 1
       for (int it = 0; it < N / BS; ++it) {
         base = base >= N ? base - N : base;
-        const double2 *fl[NP][TB];
+        const C *fl[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
           for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + (BS + base) / Gm::G : line[p][b] + BS + base;
         // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar
         // loads), or the staged x-plane in LDS
-        const double2 *gz[NP][TB];
+        const C *gz[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -353,11 +380,12 @@ struct SEntry {
 
 #define S(N, TR, WV, TB, NP, F) {N, TR, WV, TB, NP, F, &launch_s<N, TR, WV, TB, NP, F>}
 #define S3(N, TR, WV, TB, NP) S(N, TR, WV, TB, NP, 0), S(N, TR, WV, TB, NP, 1), S(N, TR, WV, TB, NP, 2)
+#define S4(N, TR, WV, TB, NP) S3(N, TR, WV, TB, NP), S(N, TR, WV, TB, NP, 3)
 
 const SEntry kSTable[] = {
     // N = 32: TR=8 (4 lanes per z-line, 2 waves per x-plane) and TR=4 (8 lanes per z-line)
-    S3(32, 8, 2, 1, 1), S3(32, 8, 2, 2, 1), S3(32, 4, 4, 2, 1), S3(32, 4, 2, 2, 1), S3(32, 4, 4, 4, 1),
-    S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S3(32, 4, 4, 2, 2), S3(32, 4, 4, 1, 2),
+    S3(32, 8, 2, 1, 1), S3(32, 8, 2, 2, 1), S4(32, 4, 4, 2, 1), S3(32, 4, 2, 2, 1), S4(32, 4, 4, 4, 1),
+    S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S4(32, 4, 4, 2, 2), S4(32, 4, 4, 1, 2),
     // small batches: more, smaller r-tiles (half and quarter x-planes)
     S3(32, 4, 2, 1, 1), S3(32, 4, 2, 1, 2), S3(32, 4, 1, 1, 1), S3(32, 4, 1, 2, 1),
     // N = 24 (rows per workgroup must divide 24^3): TR=3 x 3 waves = one x-plane,
@@ -366,8 +394,9 @@ const SEntry kSTable[] = {
     S3(24, 4, 2, 1, 1), S3(24, 4, 2, 2, 1), S3(24, 4, 2, 1, 2), S3(24, 8, 1, 1, 1), S3(24, 8, 1, 1, 2),
     S3(24, 4, 9, 1, 2),
     // N = 16: TR=4 (1 wave = 1 x-plane)
-    S3(16, 4, 1, 1, 1), S3(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S3(16, 4, 1, 2, 2),
+    S3(16, 4, 1, 1, 1), S4(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S4(16, 4, 1, 2, 2),
 };
+#undef S4
 #undef S3
 #undef S
 

@@ -637,10 +637,11 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
 
 /* Streaming kernel defaults (measured on 256-512 cells, MI355X): FP64 weights
  * favour 8 rows per lane for the two-product sum, FP32 weights 4 rows and 4
- * waves.  Batches too small to give 2048 workgroups split the summed x-planes
- * into up to 8 slices. */
+ * waves.  The FP32-product form has streaming tiles at N=16 and 32 only (the
+ * tiled kernel serves it elsewhere: unsupported candidates are skipped).
+ * Batches too small to give 2048 workgroups split the summed x-planes into up
+ * to 8 slices. */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
-  if (form == SPBTE_FORM_FP32A) return 0;
   static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}}, s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
