@@ -129,6 +129,33 @@ def test_n32_tiles_agree(spb, orc):
     op.close()
 
 
+def test_n32_fp32_products_streaming(spb, orc):
+    """The FP32-product form on the streaming kernel (every instantiated N=32
+    tile, one and two products, whole and split x-plane sums) against the FP64
+    sum: within 1e-6 relative L2 per problem, like the other reduced forms."""
+    N, Lv = 32, 6.0
+    v, eta = orc.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    rng = np.random.default_rng(33)
+    f = np.stack([_two_maxwellians(orc, v, rng).ravel() for _ in range(5)])
+    ran = 0
+    for mp in (False, True):
+        op.set_tile("fp64", mp, 4, 4, 2, 0)
+        ref = op.compute_q(f, maxpreserve=mp)
+        for t in [(4, 4, 1, 1), (4, 4, 2, 1), (4, 4, 4, 1), (4, 4, 1, 4), (4, 4, 2, 2)]:
+            try:
+                op.set_tile("fp32a", mp, *t)
+            except spb.SpbteError:
+                continue  # not instantiated for this product count
+            Q = op.compute_q(f, form="fp32a", maxpreserve=mp)
+            for b in range(len(f)):
+                assert rel_l2(Q[b], ref[b]) < 1e-6, (mp, t, b)
+            ran += 1
+    assert ran >= 5
+    op.close()
+
+
 @pytest.mark.parametrize("N", [16, 24])
 def test_maxpreserve_tiles_agree(spb, orc, N):
     """Every instantiated two-product (fused maxPreserve) tile against the oracle's
