@@ -21,7 +21,9 @@
 //  * W is read from a copy permuted for this kernel: unit (wt, c4, t, lane) holds
 //    W[row][4 c4 .. 4 c4 + 3] of row = 64 TR wt + TR lane + t, so every 16-byte
 //    wave-wide load reads 1 KiB of contiguous HBM, in consumption order.
-// Precision forms as in k_wsum.hip, FP64 arithmetic: 0 FP64 W, 1 FP32 W, 2 FP16x2 W.
+// Precision forms: 0 FP64 W, 1 FP32 W, 2 FP16x2 W with FP64 arithmetic; 3 FP32 W
+// with FP32 F, G (8-byte LDS entries, half the LDS), FP32 products and 4-step
+// FP32 partial sums added into FP64 accumulators after every W chunk.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "spbte_internal.h"
