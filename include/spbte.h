@@ -117,6 +117,25 @@ int spbte_op_store_weights_file(spbte_op *op, const char *path);
  * SPBTE_MAXPRESERVE), optionally projected (SPBTE_CONSERVE). */
 int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags);
 
+/* Allocation-free variant of spbte_compute_q.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as spbte_compute_q (bit for bit on the
+ * same tile), but every device buffer of the evaluation beyond d_f, d_g and
+ * d_Q lives in a fixed workspace of SPBTE_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by the library from the first spbte_op_create on that GPU until the
+ * process ends, and the call allocates and frees nothing.  Limits: the batch
+ * must fit the workspace (about 10 cells at N=32 and 80 at N=16 for
+ * maxPreserve in FP64); larger batches, and tables not yet resident in the
+ * layout the batch's tile streams (spbte_compute_q and spbte_autotune build
+ * them on first use), return SPBTE_ERR_ARG without launching anything.  All
+ * operators on one GPU share its workspace and each call overwrites it, so
+ * calls that overlap on one GPU (other threads, other streams) race: order
+ * them on one stream or synchronise between them. */
+#define SPBTE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form,
+                            int flags);
+
 /* Single-species projection of ncell cells in place (src/conserve.c:207-264). */
 int spbte_conserve(spbte_op *op, double *d_Q, int ncell);
 

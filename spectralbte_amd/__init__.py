@@ -93,6 +93,7 @@ def lib():
             "spbte_op_generate_weights_iso": (I, [_vp, D, D, D, D, D]),
             "spbte_op_get_weights": (I, [_vp, _dp]),
             "spbte_compute_q": (I, [_vp, _vp, _vp, _vp, I, I, I]),
+            "spbte_compute_q_noalloc": (I, [_vp, _vp, _vp, _vp, I, I, I]),
             "spbte_conserve": (I, [_vp, _vp, I]),
             "spbte_conserve_species": (I, [_vp, _vp, I]),
             "spbte_transform": (I, [_vp, _vp, _vp, I, I]),

@@ -88,6 +88,10 @@ struct spbte_op {
   /* partial sums of the split streaming kernel */
   void *d_Qp;
   size_t qp_bytes;
+  /* the fixed workspace of spbte_compute_q_noalloc on this operator's GPU */
+  void *nac;
+  size_t nac_bytes;
+  int nac_err; /* hipError_t of resolving it */
   /* bumped whenever a device buffer, the table, the species or a tile changes:
    * launch sequences captured in a graph are valid for one generation */
   unsigned gen;
@@ -293,6 +297,9 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
     spbte_op_destroy(op);
     return rc;
   }
+  /* the allocation-free entry point's workspace on this GPU (resolved once per
+   * device; a failure is reported by that entry point) */
+  op->nac_err = spbte_noalloc_scratch(&op->nac, &op->nac_bytes);
   double one = 1.0;
   rc = spbte_op_set_species(op, 1, &one);
   if (rc != SPBTE_OK) {
@@ -379,6 +386,25 @@ static int ensure_scratch(spbte_op *op, int ncell) {
   op->cap = ncell;
   op->gen++;
   return SPBTE_OK;
+}
+
+/* Buffers one evaluation works in.  The operator's own (grown on demand) for
+ * spbte_compute_q; carved from the fixed workspace for the allocation-free
+ * entry point, where nothing may be allocated: then the tables the tile streams
+ * must be resident and Qp must already hold qp_bytes. */
+typedef struct {
+  double *mom; /* 2 * ncell * SPBTE_MOM_STRIDE */
+  void *xs;    /* transform scratch */
+  void *FG;    /* transformed operands */
+  void *Qh;    /* Qhat, ncell * N^3 double2 */
+  void *Qp;    /* partial sums of the split streaming kernel (fixed sets only) */
+  size_t qp_bytes;
+  int fixed;   /* 1: allocate nothing */
+} scratch_set;
+
+static scratch_set own_scratch(const spbte_op *op) {
+  scratch_set ss = {op->d_mom, op->d_xs, op->d_FG, op->d_Qh, NULL, 0, 0};
+  return ss;
 }
 
 /* ------------------------------------------------------------------ weights */
@@ -550,7 +576,7 @@ static int storage_of(int form) { return form == SPBTE_FORM_FP32A ? SPBTE_FORM_F
 /* F/G element size of a form: FP32 complex only when the arithmetic is FP32. */
 static size_t fg_bytes(int form) { return form == SPBTE_FORM_FP32A ? 8 : 16; }
 
-static int weights_form(spbte_op *op, int form, const void **W, const float **rowscale) {
+static int weights_form(spbte_op *op, int form, int build, const void **W, const float **rowscale) {
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   *rowscale = NULL;
   const int st = storage_of(form);
@@ -559,6 +585,8 @@ static int weights_form(spbte_op *op, int form, const void **W, const float **ro
     return SPBTE_OK;
   }
   const size_t n = op->n3 * op->n3;
+  if (!op->t->dWf[st] && !build)
+    return fail(SPBTE_ERR_ARG, "the table in form %d is not resident (spbte_compute_q builds it on first use)", form);
   if (!op->t->dWf[st]) {
     size_t bytes = (st == SPBTE_FORM_FP32) ? 4 * n : 4 * n + 4 * op->n3;
     HIPCHK(hipMalloc(&op->t->dWf[st], bytes));
@@ -571,12 +599,16 @@ static int weights_form(spbte_op *op, int form, const void **W, const float **ro
 }
 
 /* The table permuted for the streaming kernel with tr rows per lane. */
-static int weights_permuted(spbte_op *op, int form, int tr, const void **Wp, const float **rowscale) {
+static int weights_permuted(spbte_op *op, int form, int tr, int build, const void **Wp, const float **rowscale) {
   const int st = storage_of(form);
   if (tr < 1 || tr > 8) return fail(SPBTE_ERR_ARG, "bad rows per lane %d", tr);
   const void *W;
-  int rc = weights_form(op, form, &W, rowscale); /* FP16x2: the row scales come from here */
+  int rc = weights_form(op, form, build, &W, rowscale); /* FP16x2: the row scales come from here */
   if (rc) return rc;
+  if (!op->t->dWp[st][tr] && !build)
+    return fail(SPBTE_ERR_ARG,
+                "the table in form %d permuted for %d rows per lane is not resident (spbte_compute_q builds it on "
+                "first use)", form, tr);
   if (!op->t->dWp[st][tr]) {
     const size_t n = op->n3 * op->n3;
     HIPCHK(hipMalloc(&op->t->dWp[st][tr], n * (st == SPBTE_FORM_FP64 ? 8 : 4)));
@@ -716,21 +748,29 @@ int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
 }
 
 /* ---------------------------------------------------------- the operator */
-static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a) {
+static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a, const scratch_set *ss) {
   const float *rs;
   if (t && t->kind >= 1) {
     const void *Wp = NULL;
-    int rc = weights_permuted(op, form, t->tr, &Wp, &rs);
+    int rc = weights_permuted(op, form, t->tr, !ss->fixed, &Wp, &rs);
     if (rc) return rc;
+    void *Qp = NULL;
     if (t->kind > 1) {
-      rc = ensure_partials(op, (size_t)t->kind * a->ncell * op->n3 * 16);
-      if (rc) return rc;
+      const size_t need = (size_t)t->kind * a->ncell * op->n3 * 16;
+      if (ss->fixed) {
+        if (need > ss->qp_bytes) return fail(SPBTE_ERR_ARG, "partial sums need %zu bytes of workspace", need);
+        Qp = ss->Qp;
+      } else {
+        rc = ensure_partials(op, need);
+        if (rc) return rc;
+        Qp = op->d_Qp;
+      }
     }
-    LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, t->kind, op->d_Qp, op->s));
+    LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, t->kind, Qp, op->s));
     return SPBTE_OK;
   }
   const void *W;
-  int rc = weights_form(op, form, &W, &rs);
+  int rc = weights_form(op, form, !ss->fixed, &W, &rs);
   if (rc) return rc;
   spbte_wsum_args b = *a;
   b.W = W;
@@ -739,31 +779,31 @@ static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_
 }
 
 static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
-                    int ncell) {
+                    int ncell, const scratch_set *ss) {
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   spbte_tile t;
   int have = pick_tile(op, ncell, form, nprod, &t);
   if (!have && form != SPBTE_FORM_FP64)
     return fail(SPBTE_ERR_UNSUPPORTED, "no reduced-precision weighted-sum kernel for N=%d", op->N);
-  spbte_wsum_args a = {NULL, F0, G0, F1, G1, op->d_Qh, ncell, nprod};
-  return launch_tile(op, form, have ? &t : NULL, &a);
+  spbte_wsum_args a = {NULL, F0, G0, F1, G1, ss->Qh, ncell, nprod};
+  return launch_tile(op, form, have ? &t : NULL, &a, ss);
 }
 
 /* One chunk of spbte_compute_q.  ev_a/ev_b (optional) bracket the weighted-sum
  * launch so its duration can be timed on the operator's stream. */
 static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
-                         hipEvent_t ev_a, hipEvent_t ev_b) {
+                         hipEvent_t ev_a, hipEvent_t ev_b, const scratch_set *ss) {
   const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
   const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
   const size_t n = (size_t)ncell * op->n3;
   const size_t esz = fg_bytes(form);
-  char *FG = (char *)op->d_FG;
+  char *FG = (char *)ss->FG;
   void *B[MAXJOBS];
   for (int q = 0; q < MAXJOBS; q++) B[q] = FG + (size_t)q * n * esz;
   spbte_xform_job jobs[MAXJOBS];
   int nj = 0;
   const void *F0, *G0, *F1 = NULL, *G1 = NULL;
-  double *mom_f = op->d_mom, *mom_g = op->d_mom + (size_t)ncell * SPBTE_MOM_STRIDE;
+  double *mom_f = ss->mom, *mom_g = ss->mom + (size_t)ncell * SPBTE_MOM_STRIDE;
   if (!mp) {
     jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};
     F0 = B[0];
@@ -788,12 +828,12 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
       G1 = B[3];
     }
   }
-  LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, fg_bytes(form) == 8, op->d_xs, op->s));
+  LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, fg_bytes(form) == 8, ss->xs, op->s));
   if (ev_a) HIPCHK(hipEventRecord(ev_a, op->s));
-  int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell);
+  int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell, ss);
   if (ev_b) HIPCHK(hipEventRecord(ev_b, op->s));
   if (rc) return rc;
-  LCHK(spbte_launch_xform_inv_real(&op->g, op->d_Qh, d_Q, ncell, op->d_xs, op->s));
+  LCHK(spbte_launch_xform_inv_real(&op->g, ss->Qh, d_Q, ncell, ss->xs, op->s));
   if (flags & SPBTE_CONSERVE) {
     if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
     LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
@@ -812,11 +852,62 @@ int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *
   for (int c0 = 0; c0 < ncell; c0 += ch) {
     int nc = (ncell - c0 < ch) ? ncell - c0 : ch;
     const size_t off = (size_t)c0 * op->n3;
+    const scratch_set ss = own_scratch(op);
     rc = compute_chunk(op, d_f + off, d_g ? d_g + off : NULL, d_Q + off, nc, form, flags | (d_g ? 0 : SPBTE_SELF), NULL,
-                       NULL);
+                       NULL, &ss);
     if (rc) return rc;
   }
   return SPBTE_OK;
+}
+
+/* Bytes rounded up to the workspace's 256-byte alignment. */
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form,
+                            int flags) {
+  if (!op || ncell < 0 || form < 0 || form >= NFORMS) return fail(SPBTE_ERR_ARG, "bad arguments");
+  if (ncell == 0) return SPBTE_OK;
+  if (!d_f || !d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (!op->nac)
+    return fail(SPBTE_ERR_HIP, "no fixed workspace on this GPU: %s", hipGetErrorString((hipError_t)op->nac_err));
+  if ((flags & SPBTE_CONSERVE) && op->Ns != 1)
+    return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
+  if (!d_g) flags |= SPBTE_SELF;
+  const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
+  const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
+  /* the tile compute_chunk will pick, and its table, which must be resident */
+  spbte_tile t;
+  const int have = pick_tile(op, ncell, form, mp ? 2 : 1, &t);
+  if (!have && form != SPBTE_FORM_FP64)
+    return fail(SPBTE_ERR_UNSUPPORTED, "no reduced-precision weighted-sum kernel for N=%d", op->N);
+  const void *W;
+  const float *rs;
+  int rc = (have && t.kind >= 1) ? weights_permuted(op, form, t.tr, 0, &W, &rs) : weights_form(op, form, 0, &W, &rs);
+  if (rc) return rc;
+  /* workspace: moments | transformed operands | Qhat | transform scratch,
+   * which also holds the split sum's partial sums (they are dead by the time
+   * the inverse transform writes it) */
+  const int nj = mp ? (self ? 3 : 4) : (self ? 1 : 2);
+  const size_t n = (size_t)ncell * op->n3;
+  const size_t b_mom = al256(sizeof(double) * 2 * (size_t)ncell * SPBTE_MOM_STRIDE);
+  const size_t b_fg = al256((size_t)nj * n * fg_bytes(form)), b_qh = al256(16 * n);
+  const size_t b_xs = 16 * n * (size_t)nj, b_qp = (have && t.kind > 1) ? 16 * n * (size_t)t.kind : 0;
+  const size_t b_x = al256(b_xs > b_qp ? b_xs : b_qp);
+  const size_t need = b_mom + b_fg + b_qh + b_x;
+  if (need > op->nac_bytes)
+    return fail(SPBTE_ERR_ARG, "a batch of %d cells at N=%d needs %zu bytes of workspace, more than the %zu of the "
+                "fixed workspace (SPBTE_NOALLOC_SCRATCH_BYTES)", ncell, op->N, need, op->nac_bytes);
+  char *base = (char *)op->nac;
+  scratch_set ss;
+  ss.mom = (double *)base;
+  ss.FG = base + b_mom;
+  ss.Qh = base + b_mom + b_fg;
+  ss.xs = base + b_mom + b_fg + b_qh;
+  ss.Qp = ss.xs;
+  ss.qp_bytes = b_x;
+  ss.fixed = 1;
+  return compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags, NULL, NULL, &ss);
 }
 
 int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
@@ -884,7 +975,8 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
     F = FG;
     G = FG + 8 * n;
   }
-  rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell);
+  const scratch_set ss = own_scratch(op);
+  rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell, &ss);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(d_Qhat, op->d_Qh, 16 * n, hipMemcpyDeviceToDevice, op->s));
   return SPBTE_OK;
@@ -913,8 +1005,10 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
   float tot = 0.f, k = 0.f, ksum = 0.f;
   if (!rc && (hipStreamSynchronize(op->s) != hipSuccess || hipEventRecord(ev[0], op->s) != hipSuccess))
     rc = fail(SPBTE_ERR_HIP, "timing set-up failed");
+  const scratch_set ss = own_scratch(op);
   for (int i = 0; i < iters && !rc; i++)
-    rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ev[2 + 2 * i], ev[3 + 2 * i]);
+    rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ev[2 + 2 * i], ev[3 + 2 * i],
+                       &ss);
   if (!rc && (hipEventRecord(ev[1], op->s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
               hipEventElapsedTime(&tot, ev[0], ev[1]) != hipSuccess))
     rc = fail(SPBTE_ERR_HIP, "timing failed: %s", hipGetErrorString(hipGetLastError()));
@@ -957,6 +1051,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
   LCHK(spbte_launch_fill_pattern(FG, esz * n * 4 / (esz == 8 ? 4 : 8), esz == 8, op->s));
   spbte_wsum_args args = {NULL, FG, FG + esz * n, np == 2 ? FG + 2 * esz * n : NULL, np == 2 ? FG + 3 * esz * n : NULL,
                           op->d_Qh, ncell, np};
+  const scratch_set ss = own_scratch(op);
   spbte_tile best = {0, 0, 0, 0};
   float best_ms = 1e30f;
   hipEvent_t a, b;
@@ -970,7 +1065,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     /* warm-up (builds the table layout it needs); a tile whose warm-up alone
      * takes twice the best time so far is not timed further */
     HIPCHK(hipEventRecord(a, op->s));
-    rc = launch_tile(op, form, t, &args);
+    rc = launch_tile(op, form, t, &args, &ss);
     HIPCHK(hipEventRecord(b, op->s));
     HIPCHK(hipEventSynchronize(b));
     if (rc) break;
@@ -979,7 +1074,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     if (warm > 2.0f * best_ms) continue;
     HIPCHK(hipEventRecord(a, op->s));
     const int reps = 2;
-    for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args);
+    for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args, &ss);
     HIPCHK(hipEventRecord(b, op->s));
     HIPCHK(hipEventSynchronize(b));
     if (rc) break;

@@ -138,6 +138,10 @@ int spbte_net_cell_norm(long n, int ncell, const double *a, double *out, hipStre
 int spbte_net_relayout(long M, int nsamp, int width, const float *src, float *dst, int to_samples, const int *idx,
                        hipStream_t s);
 
+/* k_scratch.hip: device address and size of the current GPU's fixed workspace
+ * of spbte_compute_q_noalloc (resolved once per device). */
+int spbte_noalloc_scratch(void **ptr, size_t *bytes);
+
 /* host helpers shared by the reference-compatible modules (spbte_core.c) */
 int spbte_host_lu_factor(int n, double *A, int *pivot);
 void spbte_host_lu_solve(int n, const double *LU, const int *pivot, double *b);
