@@ -66,6 +66,13 @@ int spbte_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
  * synchronous.  Also the known-size reference for HBM byte counters. */
 int spbte_device_copy(void *d_dst, const void *d_src, size_t bytes);
 int spbte_op_synchronize(spbte_op *op);
+/* Vector ALU issue ceiling, measured (no counterpart in the reference): a
+ * kernel of independent FMA chains over every CU (fp64 != 0: FP64, else
+ * unpacked FP32, the instructions this library issues; one operand
+ * wave-uniform) returns its rate in TFLOP/s (2 flops per FMA) and the shader
+ * clock during the run in GHz.  The roofline of the batched weighted sum, which
+ * is bound by FMA issue.  Synchronous, about 10 ms. */
+int spbte_valu_peak(int fp64, double *tflops, double *clock_ghz);
 
 /* One velocity grid: N nodes per axis on [-L_v, L_v] with the caller's v and
  * eta arrays (either grid convention of src/initializer.c:66-82 or :256-266). */

@@ -83,6 +83,7 @@ def lib():
             "spbte_memcpy_dtoh": (I, [_vp, _vp, S]),
             "spbte_memcpy_dtod": (I, [_vp, _vp, S]),
             "spbte_device_copy": (I, [_vp, _vp, S]),
+            "spbte_valu_peak": (I, [I, _dp, _dp]),
             "spbte_op_synchronize": (I, [_vp]),
             "spbte_op_create": (I, [ctypes.POINTER(_vp), I, D, _dp, _dp]),
             "spbte_op_destroy": (None, [_vp]),
@@ -183,6 +184,13 @@ def _p(a):
 
 def _c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def valu_peak(fp64=False):
+    """(TFLOP/s, clock GHz) of independent FMA chains on every CU (spbte_valu_peak)."""
+    t, g = ctypes.c_double(), ctypes.c_double()
+    _check(lib().spbte_valu_peak(int(bool(fp64)), ctypes.byref(t), ctypes.byref(g)))
+    return t.value, g.value
 
 
 def device_count():
