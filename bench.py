@@ -11,11 +11,16 @@ resident in HBM before timing starts.
 
 `value` = Q evaluations per second with the weight table streamed as FP32 and
 the complex products in FP32 (4-term FP32 partial sums folded into FP64
-accumulators).  Alongside: the same batch in FP64, the plain FP64 form (one cell
-per pass over the table, the reference method), the FP32 form's relative L2
-error against FP64 per problem, the 256-cell N=32 Mach-1.2 shock collision step
-(ComputeQ + projection), the roofline of the weighted-sum kernel and the CPU
-oracle timed on this host.  Progress goes to stderr, the JSON line to stdout.
+accumulators).  Alongside, each with its precision form and its error against
+FP64 stated: the same batch in FP64 and in the plain FP64 form (one cell per
+pass over the table, the reference method); the BKW relaxation at N=16; the
+256-cell N=32 Mach-1.2 shock collision step (ComputeQ + projection) in every
+form; whole-step runs (shock, mixed-N ensemble, heat transfer on the
+glancing-collision table); the anisotropic table generators; the learned
+surrogate; the roofline of the weighted-sum kernel against the published peak
+and against the issue ceiling measured in the same run; and the reference
+method's CPU restatement timed on this host.  Progress goes to stderr, the
+JSON line to stdout.
 """
 import argparse
 import json
@@ -30,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_VALU_TFLOPS = 78.6    # MI355X FP64 vector peak (FP64-arithmetic forms)
-FP32_VALU_TFLOPS = 157.3   # MI355X FP32 vector peak, packed FP32 (the FP32-product form issues unpacked FMAs)
+FP32_VALU_TFLOPS = 157.3   # MI355X FP32 vector peak as published (it counts packed FP32, which this build does not issue)
 
 
 def log(msg):
@@ -55,13 +60,12 @@ def parse():
     p.add_argument("--form", default="fp32a", choices=["fp64", "fp32", "fp16x2", "fp32a"])
     p.add_argument("--tile", default=None, help="kind,tr,lines,tb (default: autotune)")
     p.add_argument("--workload", default="ensemble", choices=["ensemble", "shock"],
-                   help="timed workload (the other one is reported as an extra unless --no-extra)")
+                   help="timed workload: the ensemble (default) or the shock collision step")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-ref", action="store_true", help="skip the FP64 comparisons")
-    p.add_argument("--no-extra", action="store_true", help="skip the secondary workload")
+    p.add_argument("--no-extra", action="store_true", help="skip every row beside the headline")
     p.add_argument("--no-runs", action="store_true", help="skip the whole-step runs (shock, mixed ensemble)")
     p.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
-    p.add_argument("--cpu-cells", type=int, default=0, help="problems in the CPU sample (0: about 15 s of work)")
     return p.parse_args()
 
 
@@ -152,24 +156,41 @@ def main():
 
     # ---- roofline of the weighted-sum kernel: bound by vector ALU issue (the
     # cell groups of a batch share each streamed piece of W through L2, so HBM
-    # traffic per flop is small); FP32 peak for the FP32-product form
+    # traffic per flop is small).  `peak` is the published vector peak; the
+    # issue ceiling of the instructions the kernel actually uses (unpacked FP32
+    # or FP64 FMA, 4 cycles per wave instruction) is measured here as well
     flops = flops_per_pair(wl.mp) * P * ncell
     sw = {"fp64": 8, "fp32": 4, "fp16x2": 4, "fp32a": 4}[a.form]
     fg = 8 if a.form == "fp32a" else 16                              # bytes per transformed complex value
     hbm_min = sw * P + ncell * n3 * (fg * (2 if not wl.mp else 4) + 16)  # W once + F/G read + Qhat written
     peak = FP32_VALU_TFLOPS if a.form == "fp32a" else FP64_VALU_TFLOPS
-    traffic = None
+    traffic, tsrc = None, "not profiled for this tile: no HBM byte count"
     tpath = os.path.join(ROOT, "profiles", "wsum_traffic.json")
+    tkey = f"{a.workload}:{a.form}:{ncell}:" + ",".join(map(str, tile))
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(f"{a.workload}:{a.form}:{ncell}:" + ",".join(map(str, tile)))
+            traffic = json.load(open(tpath)).get(tkey)
         except (OSError, ValueError):
             traffic = None
+        if traffic is not None:
+            tsrc = (f"profiles/wsum_traffic.json[{tkey}]: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this launch, "
+                    "scaled by the counters' reading for a copy kernel of known size")
     achieved = flops / (mk * 1e-3) / 1e12
+    ceil_tf, ceil_ghz = spb.valu_peak(fp64=a.form != "fp32a")
+    # per (r, c) pair and problem the kernel issues 10 FMA/MUL instructions for
+    # 18 flops (fused maxPreserve; 6 for 10 with one product), against 2 flops
+    # per FMA in the ceiling kernel
+    instr = 10.0 if wl.mp else 6.0
+    alg_ceiling = ceil_tf * flops_per_pair(wl.mp) / (2.0 * instr)
+    log(f"issue ceiling ({'FP32' if a.form == 'fp32a' else 'FP64'} FMA chains): {ceil_tf:.1f} TFLOP/s at {ceil_ghz:.2f} GHz;"
+        f" the kernel's instruction mix can reach {alg_ceiling:.1f}, it reaches {achieved:.1f}")
     roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic,
-            "kernel": "k_wsum_stream" if tile[0] == 1 else "k_wsum", "kernel_ms": round(mk, 4),
+            "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": tsrc,
+            "kernel": "k_wsum_stream" if tile[0] >= 1 else "k_wsum", "kernel_ms": round(mk, 4),
             "alg_flops_per_launch": flops,
+            "issue_ceiling_measured_tflops": round(ceil_tf, 2), "issue_ceiling_clock_ghz": round(ceil_ghz, 3),
+            "issue_ceiling_for_this_instruction_mix_tflops": round(alg_ceiling, 2),
+            "frac_of_measured_issue_ceiling": round(achieved / alg_ceiling, 4),
             "hbm_alg_min_bytes_per_launch": hbm_min,
             "hbm_alg_min_gbs": round(hbm_min / (mk * 1e-3) / 1e9, 1), "hbm_peak_gbs": HBM_PEAK_GBS}
 
@@ -183,38 +204,28 @@ def main():
         msp, kp, tp, _ = wl.run("fp64", 4, 1, ncell=1)
         plain = 1.0 / (msp * 1e-3)
         extra.update({"fp64_plain_q_per_s": round(plain, 1), "fp64_plain_tile": list(tp),
+                      "fp64_plain_ms_per_q": round(msp, 4), "fp64_plain_w_stream_ms": round(kp, 4),
                       "fp64_plain_w_stream_gbs": round(8.0 * P / (kp * 1e-3) / 1e9, 1),
                       "speedup_vs_plain_fp64": round(value / plain, 2)})
 
-    def secondary():
-        other = "shock" if a.workload == "ensemble" else "ensemble"
-        wl2 = Workload(spb, other, 256 if other == "shock" else 512)
-        ms2, mk2, t2, Q2 = wl2.run(a.form, 3, 1)
-        sec = {"workload": wl2.desc, "form": a.form, "q_per_s": round(wl2.ncell / (ms2 * 1e-3), 1), "tile": list(t2)}
-        if not a.no_ref:
-            ms3, _, t3, Q3 = wl2.run("fp64", 2, 1)
-            sec.update({"fp64_batched_q_per_s": round(wl2.ncell / (ms3 * 1e-3), 1), "fp64_tile": list(t3)})
-            sec.update(wl2.errors(Q2, Q3))
-        wl2.op.close()
-        return sec
-
     # the secondary measurements report an error instead of costing the headline line
     if not a.no_extra:
-        extra["secondary"] = guarded("secondary workload", secondary)
+        extra["c1_bkw16"] = guarded("BKW16 run", c1_bkw16, spb)
+        extra["c3_shock_forms"] = guarded("shock collision step per form", c3_shock_forms, spb)
     if not a.no_extra and not a.no_runs:
-        extra["shock_run"] = guarded("shock run", shock_run, spb, a.form)
-        extra["ensemble_mixed_run"] = guarded("mixed ensemble run", mixed_ensemble_run, spb, a.form)
+        extra["shock_run"] = guarded("shock run", shock_run, spb)
+        extra["ensemble_mixed_run"] = guarded("mixed ensemble run", mixed_ensemble_run, spb)
         extra["landau_table"] = guarded("Landau table", landau_table, spb)
         extra["glance_table"] = guarded("glancing-collision table", glance_table, spb)
-        extra["heat_glance_run"] = guarded("heat-transfer run", heat_glance_run, spb, a.form)
+        extra["heat_glance_run"] = guarded("heat-transfer run", heat_glance_run, spb)
     if not a.no_extra and not a.no_surrogate:
-        extra["surrogate"] = guarded("surrogate", surrogate_run, spb, a.form)
+        extra["surrogate"] = guarded("surrogate", surrogate_run, spb)
 
     cpu = None
     if not a.no_cpu:
-        cpu = guarded("CPU baseline", cpu_baseline, wl, a.cpu_cells)
-        if "error" in cpu:
-            cpu = None
+        res = guarded("CPU baseline", cpu_baseline, spb, wl)
+        if "error" not in res:
+            cpu, extra["cpu_baseline_table"] = res
 
     out = {"metric": "Q(f) evals/s at N=32 (batched FP32 vs plain FP64) at <=1e-6 rel err; HBM GB/s",
            "value": round(value, 1), "unit": "Q evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -229,10 +240,89 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def shock_run(spb, form, steps=3):
+def _state_change_error(f0, fa, fb):
+    """Relative L2 difference of two runs' state changes from the same start:
+    |(fa - f0) - (fb - f0)| / |fb - f0| (fb the FP64 run)."""
+    d = fb - f0
+    return float(np.linalg.norm(fa - fb) / np.linalg.norm(d))
+
+
+def c1_bkw16(spb, steps=100):
+    """The BKW relaxation (plain FP64 method): one 0D problem, N=16, L_v=5,
+    Maxwell molecules (lambda=0), the BKW state at K=0.7 (src/initializer.c:
+    123-135), Kn=1, dt=0.01, 100 Heun steps of ComputeQ_maxPreserve +
+    projection (exec/boltz.c:186-250), each step one HIP graph; and one
+    evaluation timed alone for the weight stream.  The FP64 table (134 MB) fits
+    in the 256 MB Infinity Cache, so the stream can read faster than HBM."""
+    from spectralbte_amd import workloads
+    N, Lv, dt, K = 16, 5.0, 0.01, 0.7
+    v, eta = workloads.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(0.0)
+    f0 = workloads.bkw_state(N, Lv, K)
+    run = spb.Run.ensemble(op, f0[None], order=2, dt=dt, Kn=1.0, form="fp64")
+    run.use_graph(True)
+    run.step(1)
+    run.set_state(f0[None])
+    op.sync()
+    t0 = time.perf_counter()
+    run.step(steps)
+    sec = time.perf_counter() - t0
+    m0 = run.moments()
+    C = cmat(N, v)
+    fe = run.state()[0]
+    drift = np.abs(C @ fe - C @ f0) / (np.abs(C) @ np.abs(f0))
+    run.close()
+    d_f = spb.DeviceArray.from_host(f0[None])
+    d_Q = spb.DeviceArray((1, N ** 3))
+    flags = spb.SELF | spb.MAXPRESERVE | spb.CONSERVE
+    op.compute_q_dev(d_f.ptr, None, d_Q.ptr, 1, "fp64", flags)
+    ms, mk = op.time_compute_q(d_f.ptr, None, d_Q.ptr, 1, "fp64", flags, iters=50)
+    op.close()
+    P = float(N) ** 6
+    log(f"BKW16 run: {steps} Heun steps in {sec * 1e3:.1f} ms, {2 * steps / sec:.0f} Q/s; one Q {ms * 1e3:.1f} us, "
+        f"W stream {mk * 1e3:.1f} us")
+    return {"workload": "0D-3V BKW relaxation, N=16, L_v=5, lambda=0, K=0.7, Kn=1, dt=0.01, 100 Heun steps of "
+                        "ComputeQ_maxPreserve + projection, one HIP graph per step",
+            "form": "fp64 (plain: one problem per pass over W)", "run_ms": round(sec * 1e3, 2),
+            "q_per_s": round(2 * steps / sec, 1), "single_q_us": round(ms * 1e3, 2),
+            "single_q_w_stream_us": round(mk * 1e3, 2), "w_stream_gbs": round(8.0 * P / (mk * 1e-3) / 1e9, 1),
+            "w_table_bytes": 8.0 * P,
+            "note": "the 134 MB FP64 table fits in the 256 MB Infinity Cache: the rate is not an HBM rate",
+            "final_rho_u_T": [round(float(x), 12) for x in m0[0][[0, 1, 4]]],
+            "conserved_moment_drift_max_rel": float(drift.max()),
+            "parity": "tests/test_gpu_configs.py::test_c1_bkw16_run_against_oracle (moments 1e-10 vs the oracle's driver)"}
+
+
+def c3_shock_forms(spb, ncell=256):
+    """The shock collision step (ComputeQ + projection of 256 cells, N=32,
+    L_v=8, hard spheres, Mach-1.2 profile) in every precision form, each with
+    its error against the FP64 form on the same cells.  ComputeQ keeps Q(M, M),
+    so in near-equilibrium cells Q is a small residual of large gain and loss
+    terms and FP32 rounding of W is amplified."""
+    wl = Workload(spb, "shock", ncell)
+    out, Qs = {"workload": wl.desc}, {}
+    for form in ("fp64", "fp32", "fp16x2", "fp32a"):
+        ms, mk, t, Q = wl.run(form, 3, 1)
+        Qs[form] = Q
+        out[form] = {"q_per_s": round(ncell / (ms * 1e-3), 1), "ms_per_step": round(ms, 3),
+                     "kernel_ms": round(mk, 3), "tile": list(t)}
+    for form in ("fp32", "fp16x2", "fp32a"):
+        e = wl.errors(Qs[form], Qs["fp64"])
+        out[form].update(e)
+        out[form]["meets_1e-6"] = bool(e["rel_l2_err_vs_fp64_batch"] <= 1e-6)
+    out["fp64"]["meets_1e-6"] = True
+    out["fp64"]["parity"] = "tests/test_gpu_configs.py::test_shock_cells_n32_against_oracle (1e-12 vs the oracle)"
+    wl.op.close()
+    return out
+
+
+def shock_run(spb, steps=3):
     """Whole steps of the 1D-3V Mach-1.2 shock (256 cells, N=32, L_v=8, lambda=1,
     Kn=0.1, order 2, dt=0.002): advectTwo, Heun collision step (2 Q per cell),
-    advectTwo, replayed as one captured HIP graph per step."""
+    advectTwo, replayed as one captured HIP graph per step; in FP64 (the form
+    that meets 1e-6 on ComputeQ near equilibrium) and in the FP32-product form,
+    with that run's state change against the FP64 run's."""
     from spectralbte_amd import workloads
     N, Lv, nX = 32, 8.0, 256
     v, eta = workloads.grid_1d(N, Lv)
@@ -240,22 +330,27 @@ def shock_run(spb, form, steps=3):
     op.generate_weights_iso(1.0)
     x, dx = _mesh(nX, 1.0, 2)
     f0 = workloads.shock_profile(N, Lv, nX)
-    run = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form=form)
-    op.autotune(nX, form, False)
-    run.use_graph(True)
-    run.step(1)
-    t0 = time.perf_counter()
-    run.step(steps)
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    m = run.moments()
-    ok = bool(np.all(np.isfinite(m)))
-    run.close()
+    res, states = {}, {}
+    for form in ("fp64", "fp32a"):
+        run = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form=form)
+        op.autotune(nX, form, False)
+        run.use_graph(True)
+        run.step(1)
+        run.set_state(f0)
+        op.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        states[form] = run.state()
+        run.close()
+        log(f"shock run {form}: {ms:.2f} ms/step (graph), {2 * nX / ms * 1e3:.1f} Q/s")
+        res[form] = {"ms_per_step": round(ms, 3), "q_per_s": round(2 * nX / ms * 1e3, 1),
+                     "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": bool(np.all(np.isfinite(states[form])))}
+    res["fp32a"]["state_change_rel_err_vs_fp64"] = _state_change_error(f0, states["fp32a"], states["fp64"])
     op.close()
-    log(f"shock run {form}: {ms:.2f} ms/step (graph), {2 * nX / ms * 1e3:.1f} Q/s")
     return {"workload": "1D-3V Mach-1.2 shock, 256 cells, N=32, L_v=8, lambda=1, Kn=0.1, order 2 (advectTwo + "
                         "Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
-            "form": form, "ms_per_step": round(ms, 3), "q_per_s": round(2 * nX / ms * 1e3, 1),
-            "steps_timed": steps, "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": ok}
+            "steps_timed": steps, **res}
 
 
 def landau_table(spb, N=32, Lv=7.0, lam=1.0):
@@ -296,13 +391,16 @@ def glance_table(spb, N=32, Lv=7.0, lam=1.0, glance=0.01):
             "seconds": round(sec, 2), "entries_per_s": round(n / sec, 1)}
 
 
-def heat_glance_run(spb, form, steps=3):
+def heat_glance_run(spb, steps=3):
     """Whole steps of the 1D-3V heat transfer between diffuse walls at T=1 and
     T=2 with the anisotropic glancing-collision table (grazing parameter 0.1):
     128 cells, N=32, L_v=7, lambda=1, Kn=0.05, order 1, a Maxwellian at rho=1,
-    u=0, T=1.5 with 1% seeded density noise.  The table carries no trapezoid or
-    h_eta^3 factors, as the reference writes it, so Q is about 1e4 times the
-    isotropic table's and the explicit step is stable near dt=1e-6."""
+    u=0, T=1.5 with 1% seeded density noise, autotuned tiles; in FP64 and in
+    the FP32-product form, with the latter's error against FP64 on the
+    collision operator of the initial state and on the runs' state change.
+    The table carries no trapezoid or h_eta^3 factors, as the reference writes
+    it, so Q is about 1e4 times the isotropic table's and the explicit step is
+    stable near dt=1e-6 (2000 steps cover 2e-3 time units)."""
     from spectralbte_amd import workloads
     N, Lv, nX = 32, 7.0, 128
     v, eta = workloads.grid_1d(N, Lv)
@@ -312,21 +410,32 @@ def heat_glance_run(spb, form, steps=3):
     noise = 1.0 + 0.01 * np.random.default_rng(128).standard_normal(nX)
     V2 = (v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2).ravel()
     f0 = np.stack([r * np.exp(-V2 / 1.5) / (1.5 * np.pi) ** 1.5 for r in noise])  # src/initializer.c:366
-    run = spb.Run.mesh(op, f0, x, dx, order=1, dt=1e-6, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
-    op.autotune(nX, form, False)
-    run.use_graph(True)
-    run.step(1)
-    t0 = time.perf_counter()
-    run.step(steps)
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    ok = bool(np.all(np.isfinite(run.moments())))
-    run.close()
+    res, states, qs = {}, {}, {}
+    for form in ("fp64", "fp32a"):
+        tile = op.autotune(nX, form, False)
+        qs[form] = op.compute_q(f0, form=form, conserve=True)
+        run = spb.Run.mesh(op, f0, x, dx, order=1, dt=1e-6, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
+        run.use_graph(True)
+        run.step(1)
+        run.set_state(f0)
+        op.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        states[form] = run.state()
+        run.close()
+        log(f"heat-transfer run (glance 0.1) {form}: {ms:.2f} ms/step, {nX / ms * 1e3:.1f} Q/s")
+        res[form] = {"ms_per_step": round(ms, 3), "q_per_s": round(nX / ms * 1e3, 1), "tile": list(tile),
+                     "projected_2000_steps_s": round(2000 * ms / 1e3, 1),
+                     "finite": bool(np.all(np.isfinite(states[form])))}
+    e_q = float(np.linalg.norm(qs["fp32a"] - qs["fp64"]) / np.linalg.norm(qs["fp64"]))
+    res["fp32a"]["q_rel_l2_err_vs_fp64_initial_state"] = e_q
+    res["fp32a"]["state_change_rel_err_vs_fp64"] = _state_change_error(f0, states["fp32a"], states["fp64"])
+    res["fp32a"]["meets_1e-6"] = bool(e_q <= 1e-6)
     op.close()
-    log(f"heat-transfer run (glance 0.1) {form}: {ms:.2f} ms/step, {nX / ms * 1e3:.1f} Q/s")
     return {"workload": "1D-3V heat transfer, 128 cells, N=32, L_v=7, lambda=1, glancing-collision table "
                         "(glance 0.1), Kn=0.05, order 1, diffuse walls T=1 and T=2, dt=1e-6, whole step as one HIP graph",
-            "form": form, "ms_per_step": round(ms, 3), "q_per_s": round(nX / ms * 1e3, 1), "steps_timed": steps,
-            "projected_2000_steps_s": round(2000 * ms / 1e3, 1), "finite": ok}
+            "steps_timed": steps, **res}
 
 
 def _mesh(nX, length, order):
@@ -336,13 +445,14 @@ def _mesh(nX, length, order):
     return x, np.full(nX + 2 * order, d)
 
 
-def mixed_ensemble_run(spb, form, ncell=512):
+def mixed_ensemble_run(spb, form="fp32a", ncell=512, steps=2):
     """The whole 0D ensemble: 512 problems at N=24 and 512 at N=32 (L_v=6,
     lambda=1), Heun steps of dt=0.005 (2 Q per problem per step), timed with the
     two grids' runs queued on their own streams (overlapping) and one after the
-    other; the faster schedule is reported."""
+    other; the faster schedule is reported, with the FP32-product runs' state
+    change against FP64 runs of the same steps."""
     from spectralbte_amd import workloads
-    runs, ops = [], []
+    runs, ops, f0s = [], [], []
     for N, seed in ((24, 0x5EED0024), (32, 0x5EED0032)):
         v, eta = workloads.grid_0d(N, 6.0)
         op = spb.Operator(N, 6.0, v, eta)
@@ -354,13 +464,18 @@ def mixed_ensemble_run(spb, form, ncell=512):
         r.step(1)
         runs.append(r)
         ops.append(op)
-    steps = 2
+        f0s.append(f0)
+    for r, f0 in zip(runs, f0s):
+        r.set_state(f0)
     t0 = time.perf_counter()
     for r in runs:
         r.step(steps, sync=False)
     for op in ops:
         op.sync()
     ms_overlap = (time.perf_counter() - t0) * 1e3 / steps
+    st32 = [r.state() for r in runs]
+    for r, f0 in zip(runs, f0s):
+        r.set_state(f0)
     t0 = time.perf_counter()
     for r in runs:  # one grid after the other
         r.step(steps)
@@ -372,16 +487,21 @@ def mixed_ensemble_run(spb, form, ncell=512):
         r.step(1)
         per[f"N{N}_ms_per_step"] = round((time.perf_counter() - t0) * 1e3, 2)
         r.close()
-    for op in ops:
+    errs = {}
+    for N, op, f0, st in zip((24, 32), ops, f0s, st32):
+        r = spb.Run.ensemble(op, f0, order=2, dt=0.005, Kn=1.0, form="fp64")
+        r.step(steps)
+        errs[f"N{N}_state_change_rel_err_vs_fp64"] = _state_change_error(f0, st, r.state())
+        r.close()
         op.close()
     q = 2 * 2 * ncell / ms * 1e3
-    log(f"mixed ensemble {form}: {ms:.2f} ms/step for 2x{ncell} problems, {q:.1f} Q/s")
+    log(f"mixed ensemble {form}: {ms:.2f} ms/step for 2x{ncell} problems, {q:.1f} Q/s; {errs}")
     return {"workload": f"0D-3V ensemble, {ncell} problems at N=24 + {ncell} at N=32, L_v=6, lambda=1, Heun dt=0.005, "
                         "ComputeQ_maxPreserve + projection, the faster of overlapped and sequential grids",
-            "form": form, "ms_per_step": round(ms, 2), "q_per_s": round(q, 1), **per}
+            "form": form, "ms_per_step": round(ms, 2), "q_per_s": round(q, 1), **per, **errs}
 
 
-def surrogate_run(spb, form, ncell=4096, train_cells=2048, steps=2500, batch=64):
+def surrogate_run(spb, form="fp64", ncell=4096, train_cells=2048, steps=2500, batch=64):
     """Learned surrogate on the N=32, L_v=8 hard-sphere grid: trained on the GPU
     on exact Q of shock-profile cells, judged on held-out cells, then timed on a
     4096-cell batch alone and inside the checked collision step (every 8th cell
@@ -442,31 +562,94 @@ def surrogate_run(spb, form, ncell=4096, train_cells=2048, steps=2500, batch=64)
             "checked_step_max_sampled_err": float(err), "check": "every 8th cell, tol 1e-3, floor 0.1 x max|Q|"}
 
 
-def cpu_baseline(wl, ncells):
-    """The CPU oracle (C + OpenMP restatement of the reference loop nest, FP64) on
-    a bounded sample of the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spb, wl):
+    """The reference method's CPU restatement (oracle/spbte_oracle.c: the
+    output-row-parallel FP64 loop nest of src/collisions.c:127-165 with OpenMP
+    over rows, W as the reference holds it, the twiddled transform and the 5x5
+    projection), built here -O3 -march=native, timed on this host with all the
+    threads this process may use and with 1 thread: ComputeQ and
+    ComputeQ_maxPreserve (three passes over W, as the reference) + projection
+    at N=16 (the BKW grid), 24 and 32 (the ensemble grids), on the workloads'
+    own states.  Returns (the headline workload's figure, the table)."""
+    import tempfile
+
     import oracle
-    oracle.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, 16))
-    N, v, eta, f = wl.N, wl.v, wl.eta, wl.f
-    W = wl.op.get_weights()
-    cons = oracle.Conserver(N, v)
-    fn = oracle.compute_q_maxpreserve if wl.mp else oracle.compute_q
-    t0 = time.time()
-    cons(fn(N, v, eta, W, f[0], f[0], nthreads=threads).ravel())
-    t1 = time.time() - t0
-    n = ncells or max(1, min(16, int(15.0 / max(t1, 1e-3))))
-    t0 = time.time()
-    for b in range(1, n + 1):
-        cons(fn(N, v, eta, W, f[b % len(f)], f[b % len(f)], nthreads=threads).ravel())
-    dt = time.time() - t0
-    del W
-    log(f"CPU oracle: {n} problems in {dt:.1f}s on {threads} threads")
-    op = "ComputeQ_maxPreserve (three passes over W, as the reference)" if wl.mp else "ComputeQ"
-    return {"value": round(n / dt, 4), "unit": "Q evals/s", "cores": threads, "kind": "port",
-            "sample": f"{n} problems of the same workload ({op} + projection, N=32, FP64) through "
-                      f"oracle/spbte_oracle.c with {threads} OpenMP threads over output rows, {dt:.1f} s"}
+    from spectralbte_amd import workloads
+    tmp = tempfile.mkdtemp(prefix="spbte_cpu_")
+    try:
+        oracle.use_library(oracle.build_native(tmp))
+        build = "gcc -O3 -march=native -fopenmp (compiled on this host)"
+    except Exception as e:  # noqa: BLE001 -- fall back to the portable build
+        log(f"native oracle build failed ({e}); timing the portable build")
+        oracle.build()
+        build = "gcc -O2 -fopenmp (portable build)"
+    allt = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30))
+    rows = []
+    cases = [(16, 5.0, 0.0, "BKW K=0.7"), (24, 6.0, 1.0, "two-Maxwellian ensemble"),
+             (32, 6.0, 1.0, "two-Maxwellian ensemble")]
+    headline = None
+    for N, Lv, lam, what in cases:
+        if N == wl.N and Lv == wl.Lv:
+            W, v, eta, f = wl.op.get_weights(), wl.v, wl.eta, wl.f[:3]
+        else:
+            v, eta = workloads.grid_0d(N, Lv)
+            op = spb.Operator(N, Lv, v, eta)
+            op.generate_weights_iso(lam)
+            W = op.get_weights()
+            op.close()
+            f = (workloads.bkw_state(N, Lv, 0.7)[None].repeat(3, 0) if lam == 0.0
+                 else workloads.two_maxwellian_ensemble(N, Lv, 3, seed=0x5EED0000 + N))
+        cons = oracle.Conserver(N, v)
+        for mp in (False, True):
+            fn = oracle.compute_q_maxpreserve if mp else oracle.compute_q
+            for threads in (allt, 1):
+                reps = 3 if (threads > 1 or N <= 16) else 1
+                t0 = time.perf_counter()
+                for b in range(reps):
+                    cons(fn(N, v, eta, W, f[b], f[b], nthreads=threads).ravel())
+                dt = time.perf_counter() - t0
+                row = {"N": N, "L_v": Lv, "lambda": lam, "state": what,
+                       "operator": "ComputeQ_maxPreserve" if mp else "ComputeQ", "threads": threads,
+                       "evaluations": reps, "seconds": round(dt, 3), "q_per_s": round(reps / dt, 4)}
+                rows.append(row)
+                log(f"CPU N={N} {row['operator']} {threads} thread(s): {reps / dt:.3f} Q/s")
+                if N == wl.N and mp == wl.mp and threads == allt:
+                    headline = row
+        del W
+    shutil_rm(tmp)
+    table = {"what": "reference-method CPU restatement (oracle/spbte_oracle.c), not the reference binary",
+             "build": build, "cpu_model": _cpu_model(), "logical_cpus_visible": len(os.sched_getaffinity(0)),
+             "rows": rows}
+    sh = [r for r in rows if r["N"] == 32 and r["operator"] == "ComputeQ" and r["threads"] == allt]
+    if sh:
+        table["shock_step_extrapolated_s"] = round(2 * 256 / sh[0]["q_per_s"], 1)
+        table["shock_step_note"] = ("one 1D shock step (256 cells, 2 ComputeQ each) extrapolated from the N=32 "
+                                    "ComputeQ rate at L_v=6; the transport stages are not timed")
+    if headline is None:  # a headline grid outside the table: the N=32 maxPreserve row stands in
+        headline = [r for r in rows if r["N"] == 32 and r["operator"] == "ComputeQ_maxPreserve"
+                    and r["threads"] == allt][0]
+    op_name = "ComputeQ_maxPreserve (three passes over W, as the reference)" if wl.mp else "ComputeQ"
+    cpu = {"value": headline["q_per_s"], "unit": "Q evals/s", "cores": headline["threads"], "kind": "port",
+           "sample": f"{headline['evaluations']} problems of the headline workload ({op_name} + projection, N=32, "
+                     f"FP64) through oracle/spbte_oracle.c, {build}, {headline['threads']} OpenMP threads over "
+                     f"output rows on {table['cpu_model']}, {headline['seconds']} s; the 1-thread and N=16/24 "
+                     "figures are in cpu_baseline_table"}
+    return cpu, table
+
+
+def shutil_rm(path):
+    import shutil
+    shutil.rmtree(path, ignore_errors=True)
 
 
 if __name__ == "__main__":

@@ -36,6 +36,24 @@ def build(force=False):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
 
 
+def build_native(outdir):
+    """Compile spbte_oracle.c for the host it runs on (-O3 -march=native
+    -fopenmp) into outdir; returns the library's path.  This is the
+    CPU-baseline build: the tests use the portable liboracle.so."""
+    path = os.path.join(outdir, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-std=gnu99", "-fPIC", "-fopenmp", "-shared", "-o", path,
+                    os.path.join(_HERE, "spbte_oracle.c"), "-lm"], check=True)
+    return path
+
+
+def use_library(path):
+    """Load the oracle functions from `path` (a build of spbte_oracle.c) from now on."""
+    global _LIB_PATH, _lib
+    _LIB_PATH = path
+    _lib = None
+    return lib()
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -80,3 +80,12 @@ def two_maxwellian_ensemble(N, Lv, n, seed):
         f = sum(a[k] * maxwellian(v, 1.0, rng.uniform(-1.5, 1.5, 3), rng.uniform(0.5, 2.0)).ravel() for k in range(2))
         out[i] = f / (W3 @ f)
     return out
+
+
+def bkw_state(N, Lv, K):
+    """The BKW exact solution at K on the 0D grid (src/initializer.c:123-135,
+    unit density, zero bulk velocity, unit temperature):
+    (2 pi K)^-3/2 exp(-|v|^2/(2K)) [(5K-3)/(2K) + (1-K)/(2K^2) |v|^2].  (N^3,)"""
+    v, _ = grid_0d(N, Lv)
+    V2 = (v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2).ravel()
+    return (2 * np.pi * K) ** -1.5 * np.exp(-V2 / (2 * K)) * ((5 * K - 3) / (2 * K) + (1 - K) / (2 * K * K) * V2)
