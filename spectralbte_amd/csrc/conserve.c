@@ -120,7 +120,7 @@ void conserveAllMoments(double **Q) {
   for (int a = 0; a < Ns * Ns; a++)
     if ((e = hipMemcpyAsync(d_Q + a * n3, Q[a], sizeof(double) * n3, hipMemcpyHostToDevice, stream)) != hipSuccess)
       die("conserveAllMoments", e);
-  int le = spbte_launch_project(&grid, Ns, masses, CCt, pivot, d_Q, 1, stream);
+  int le = spbte_launch_project(&grid, Ns, masses, CCt, pivot, d_Q, 1, NULL, stream);
   if (le != 0) die("conserveAllMoments", (hipError_t)le);
   for (int a = 0; a < Ns * Ns; a++)
     if ((e = hipMemcpyAsync(Q[a], d_Q + a * n3, sizeof(double) * n3, hipMemcpyDeviceToHost, stream)) != hipSuccess)

@@ -15,11 +15,17 @@
 //   pass B, one workgroup per (y index b, cell, input): the N^2 values
 //     T[:][b][:] are staged in LDS, transformed along x, scaled and written in
 //     the consumer's format (FP64 complex, FP32 complex, or the real part).
-// Work is O(3 N^4) complex MACs per cube: 0.2% of the O(N^6) weighted sum at
-// N=32, so the matrix form (which also serves odd and non-power-of-two N, as
-// FFTW does for the reference) costs nothing measurable.
+// For N = 2^a 3^b up to 64 (every grid of the benchmark configurations) each
+// axis is a Stockham FFT in LDS instead of the matrix: M factors as
+// diag(q) DFT diag(p) with p_j = exp(i s j L_start delta) w_j (the reference's
+// pre-twiddle and trapezoid weight) and q_k = exp(i s L_end a_k) (its
+// post-twiddle), so pass A multiplies the plane by p(i) p(j) p(k), runs the z
+// and y FFTs, and pass B runs the x FFT and multiplies by the prefactor and
+// q(a) q(b) q(c): O(N^3 log N) per cube, as FFTW's.  Other N keep the matrix
+// form (O(3 N^4) complex MACs per cube), which serves any N as FFTW does.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <type_traits>
 #include "spbte_internal.h"
 
 namespace {
@@ -133,6 +139,209 @@ __global__ void __launch_bounds__(XT) k_xform_b(int N, int ncell, const double2 
       if (OUT == 0) ((double2 *)dst)[o] = acc;
       else ((float2 *)dst)[o] = make_float2((float)acc.x, (float)acc.y);
     }
+  }
+}
+
+// ------------------------------------------------------------------- FFTs
+// Stage plan of a length-n FFT: radix-4 stages, then radix 2, then radix 3.
+constexpr int fft_nstages(int n) {
+  int s = 0;
+  while (n % 4 == 0) { n /= 4; ++s; }
+  while (n % 2 == 0) { n /= 2; ++s; }
+  while (n % 3 == 0) { n /= 3; ++s; }
+  return n == 1 ? s : -1;
+}
+constexpr int fft_radix(int n, int st) {
+  int s = 0;
+  while (n % 4 == 0) { if (s == st) return 4; n /= 4; ++s; }
+  while (n % 2 == 0) { if (s == st) return 2; n /= 2; ++s; }
+  while (n % 3 == 0) { if (s == st) return 3; n /= 3; ++s; }
+  return 0;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+// R-point DFT y_t = sum_r v_r exp(dsgn 2 pi i r t / R) in place.
+template <int R>
+__device__ __forceinline__ void butterfly(double2 (&v)[R], double dsgn) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = make_double2(a.x + b.x, a.y + b.y);
+    v[1] = make_double2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    // J = exp(dsgn i pi / 2) = dsgn i
+    const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    const double2 jd13 = make_double2(-dsgn * d13.y, dsgn * d13.x);  // J (v1 - v3)
+    v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+    v[1] = make_double2(d02.x + jd13.x, d02.y + jd13.y);
+    v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+    v[3] = make_double2(d02.x - jd13.x, d02.y - jd13.y);
+  } else {
+    // omega = exp(dsgn 2 pi i / 3) = -1/2 + dsgn i sqrt(3)/2
+    const double h = 0.86602540378443864676 * dsgn;
+    const double2 a = v[0], b = v[1], c = v[2];
+    const double2 sbc = make_double2(b.x + c.x, b.y + c.y), dbc = make_double2(b.x - c.x, b.y - c.y);
+    const double2 m = make_double2(a.x - 0.5 * sbc.x, a.y - 0.5 * sbc.y);
+    v[0] = make_double2(a.x + sbc.x, a.y + sbc.y);
+    v[1] = make_double2(m.x - h * dbc.y, m.y + h * dbc.x);
+    v[2] = make_double2(m.x + h * dbc.y, m.y - h * dbc.x);
+  }
+}
+
+// One Stockham stage of radix R (span Ns = product of the earlier radices) on
+// nseq length-N sequences in LDS: element m of sequence q at q * qs + m * ms
+// of src, results at the same places of dst.  tw[m] = exp(dsgn 2 pi i m / N).
+template <int N, int R>
+__device__ __forceinline__ void fft_stage(const double2 *src, double2 *dst, int qs, int ms, int nseq, int Ns,
+                                          const double2 *tw, double dsgn) {
+  constexpr int NB = N / R;
+  const int step = N / (Ns * R);
+  for (int it = threadIdx.x; it < nseq * NB; it += XT) {
+    const int q = it / NB, j = it - q * NB;
+    const int k = j % Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[q * qs + (j + r * NB) * ms];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * r * step]);
+    butterfly<R>(v, dsgn);
+    const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[q * qs + (o + r * Ns) * ms] = v[r];
+  }
+}
+
+// All stages of a length-N FFT along one axis of the plane in a[]; b[] is the
+// other buffer.  Returns the buffer holding the result.
+template <int N>
+__device__ double2 *fft_axis(double2 *a, double2 *b, int qs, int ms, const double2 *tw, double dsgn) {
+  constexpr int NS = fft_nstages(N);
+  static_assert(NS > 0, "N must be 2^a 3^b");
+  int Ns = 1;
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    const int R = fft_radix(N, st);
+    if (R == 4) fft_stage<N, 4>(a, b, qs, ms, N, Ns, tw, dsgn);
+    else if (R == 2) fft_stage<N, 2>(a, b, qs, ms, N, Ns, tw, dsgn);
+    else fft_stage<N, 3>(a, b, qs, ms, N, Ns, tw, dsgn);
+    __syncthreads();
+    Ns *= R;
+    double2 *t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// Per-axis factors of one direction: p[N], q[N], tw[N] (double2 each), in the
+// grid's d_fft block as forward p, q, tw, then inverse p, q, tw.
+struct FftTabs {
+  double2 p[64], q[64], tw[64];
+};
+
+template <int N>
+__device__ void load_tabs(FftTabs &t, const double2 *__restrict__ g) {
+  for (int e = threadIdx.x; e < 3 * N; e += XT) {
+    const int which = e / N, m = e - which * N;
+    const double2 x = g[e];
+    if (which == 0) t.p[m] = x;
+    else if (which == 1) t.q[m] = x;
+    else t.tw[m] = x;
+  }
+}
+
+// Pass A with FFTs, real inputs (with the optional Maxwellian split): x-plane i
+// of (cell, job) times p(i) p(j) p(k), then z and y FFTs, to scratch.
+template <int N>
+__global__ void __launch_bounds__(XT) k_fft_a_real(XformJobs jobs, int ncell, const double2 *__restrict__ tabs,
+                                                   const double *__restrict__ v, double2 *__restrict__ scratch, double dsgn) {
+  __shared__ double2 A[N * N], B[N * N];
+  __shared__ FftTabs t;
+  constexpr int n2 = N * N;
+  constexpr size_t n3 = (size_t)n2 * N;
+  const int i = blockIdx.x, cell = blockIdx.y, jb = blockIdx.z;
+  const spbte_xform_job &job = jobs.j[jb];
+  load_tabs<N>(t, tabs);
+  __syncthreads();
+  const double *mom = job.mom ? job.mom + (size_t)cell * SPBTE_MOM_STRIDE : nullptr;
+  const double2 pi_ = t.p[i];
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int j = e / N, k = e - j * N;
+    double val;
+    if (job.mode == 0) val = job.src[cell * n3 + (size_t)i * n2 + e];
+    else if (job.mode == 1) val = job.src[cell * n3 + (size_t)i * n2 + e] - maxwellian_at(mom, v, i, j, k);
+    else val = maxwellian_at(mom, v, i, j, k);
+    const double2 pjk = cmul(cmul(pi_, t.p[j]), t.p[k]);
+    A[e] = make_double2(val * pjk.x, val * pjk.y);
+  }
+  __syncthreads();
+  double2 *r = fft_axis<N>(A, B, N, 1, t.tw, dsgn);        // z: sequences j (stride N), elements k
+  double2 *o = (r == A) ? B : A;
+  r = fft_axis<N>(r, o, 1, N, t.tw, dsgn);                 // y: sequences k, elements j (stride N)
+  double2 *dst = scratch + ((size_t)jb * ncell + cell) * n3 + (size_t)i * n2;
+  for (int e = threadIdx.x; e < n2; e += XT) dst[e] = r[e];
+}
+
+// Pass A with FFTs, complex FP64 inputs.
+template <int N>
+__global__ void __launch_bounds__(XT) k_fft_a_cplx(const double2 *__restrict__ in, int ncell,
+                                                   const double2 *__restrict__ tabs, double2 *__restrict__ scratch,
+                                                   double dsgn) {
+  __shared__ double2 A[N * N], B[N * N];
+  __shared__ FftTabs t;
+  constexpr int n2 = N * N;
+  constexpr size_t n3 = (size_t)n2 * N;
+  const int i = blockIdx.x, cell = blockIdx.y;
+  load_tabs<N>(t, tabs);
+  __syncthreads();
+  const double2 pi_ = t.p[i];
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int j = e / N, k = e - j * N;
+    A[e] = cmul(in[cell * n3 + (size_t)i * n2 + e], cmul(cmul(pi_, t.p[j]), t.p[k]));
+  }
+  __syncthreads();
+  double2 *r = fft_axis<N>(A, B, N, 1, t.tw, dsgn);
+  double2 *o = (r == A) ? B : A;
+  r = fft_axis<N>(r, o, 1, N, t.tw, dsgn);
+  double2 *dst = scratch + (size_t)cell * n3 + (size_t)i * n2;
+  for (int e = threadIdx.x; e < n2; e += XT) dst[e] = r[e];
+}
+
+// Pass B with FFTs: the y index b plane (x, z) of scratch, x FFT, times
+// pre q(a) q(b) q(c); OUT = 0 double2, 1 float2, 2 real part.
+template <int N, int OUT>
+__global__ void __launch_bounds__(XT) k_fft_b(int ncell, const double2 *__restrict__ tabs, double pre,
+                                              const double2 *__restrict__ scratch, XformJobs jobs, void *out0,
+                                              double dsgn) {
+  __shared__ double2 A[N * N], B[N * N];
+  __shared__ FftTabs t;
+  constexpr int n2 = N * N;
+  constexpr size_t n3 = (size_t)n2 * N;
+  const int b = blockIdx.x, cell = blockIdx.y, jb = blockIdx.z;
+  load_tabs<N>(t, tabs);
+  const double2 *src = scratch + ((size_t)jb * ncell + cell) * n3;
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int i = e / N, c = e - i * N;
+    A[e] = src[(size_t)i * n2 + b * N + c];
+  }
+  __syncthreads();
+  double2 *r = fft_axis<N>(A, B, 1, N, t.tw, dsgn);        // x: sequences c, elements i (stride N)
+  void *dst = jobs.j[jb].dst ? jobs.j[jb].dst : out0;
+  const double2 qb = t.q[b];
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int a = e / N, c = e - a * N;
+    const size_t o = cell * n3 + (size_t)a * n2 + b * N + c;
+    double2 f = cmul(cmul(cmul(t.q[a], qb), t.q[c]), r[e]);
+    f.x *= pre;
+    f.y *= pre;
+    if (OUT == 0) ((double2 *)dst)[o] = f;
+    else if (OUT == 1) ((float2 *)dst)[o] = make_float2((float)f.x, (float)f.y);
+    else ((double *)dst)[o] = f.x;
   }
 }
 
@@ -266,10 +475,181 @@ __global__ void __launch_bounds__(XT) k_project(int N, int Ns, const double *__r
   }
 }
 
+// ---- the same moments and projection for small batches: one workgroup per
+// (x-plane, cell) sums its plane into a partial (PS values), and a second pass
+// combines the N partials of a cell, so one cell keeps N workgroups busy
+// instead of one.
+constexpr int PS = 5;
+
+// Partial sums of w f, w v_x f, w v_y f, w v_z f, w |v|^2 f over x-plane i.
+__global__ void __launch_bounds__(XT) k_moments_part(int N, const double *__restrict__ f, const double *__restrict__ v,
+                                                     const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double red[PS * (XT / 64)];
+  const int i = blockIdx.x, cell = blockIdx.y;
+  const int n2 = N * N;
+  const double *fp = f + (size_t)cell * n2 * N + (size_t)i * n2;
+  const double vi = v[i], wi = w[i];
+  double s[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int j = e / N, k = e - j * N;
+    const double ww = wi * w[j] * w[k] * fp[e];
+    s[0] += ww;
+    s[1] += vi * ww;
+    s[2] += v[j] * ww;
+    s[3] += v[k] * ww;
+    s[4] += (vi * vi + v[j] * v[j] + v[k] * v[k]) * ww;
+  }
+  block_sum<PS>(s, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < PS; ++q) part[((size_t)cell * N + i) * PS + q] = s[q];
+}
+
+// rho, u, T and the Maxwellian prefactor from a cell's N partials
+// (src/momentRoutines.c:58-72, :116-142, :168-183).  The temperature's
+// sum of w |v - u|^2 f is taken as sum w |v|^2 f - |sum w v f|^2 / sum w f,
+// which equals the reference's two-pass sum up to rounding.
+__global__ void __launch_bounds__(64) k_moments_fin(int N, int ncell, const double *__restrict__ part, double hv,
+                                                    double mass, double kb, double *__restrict__ mom) {
+  const int cell = blockIdx.x * 64 + threadIdx.x;
+  if (cell >= ncell) return;
+  double s[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = 0; i < N; ++i)
+    for (int q = 0; q < PS; ++q) s[q] += part[((size_t)cell * N + i) * PS + q];
+  const double dv3 = hv * hv * hv;
+  const double rho = mass * dv3 * s[0];
+  const double ux = s[1] / s[0], uy = s[2] / s[0], uz = s[3] / s[0];
+  const double c2 = s[4] - (s[1] * ux + s[2] * uy + s[3] * uz);
+  const double T = (mass * mass / kb) * dv3 * c2 / (3.0 * rho);
+  double *m = mom + (size_t)cell * SPBTE_MOM_STRIDE;
+  m[0] = rho;
+  m[1] = ux;
+  m[2] = uy;
+  m[3] = uz;
+  m[4] = T;
+  m[5] = rho * pow(0.5 / (M_PI * T), 1.5);
+}
+
+// One species: partial sums of p {1, v_x, v_y, v_z, |v|^2/2} Q over x-plane i,
+// p = w h_v^3 m (the rows of C, src/conserve.c:282-296).
+__global__ void __launch_bounds__(XT) k_proj_part(int N, const double *__restrict__ v, const double *__restrict__ w,
+                                                  double pm, const double *__restrict__ Q, double *__restrict__ part) {
+  __shared__ double red[PS * (XT / 64)];
+  const int i = blockIdx.x, cell = blockIdx.y;
+  const int n2 = N * N;
+  const double *qp = Q + (size_t)cell * n2 * N + (size_t)i * n2;
+  const double vi = v[i], wi = w[i] * pm;
+  double s[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int j = e / N, k = e - j * N;
+    const double p = wi * w[j] * w[k] * qp[e];
+    s[0] += p;
+    s[1] += p * vi;
+    s[2] += p * v[j];
+    s[3] += p * v[k];
+    s[4] += p * 0.5 * (vi * vi + v[j] * v[j] + v[k] * v[k]);
+  }
+  block_sum<PS>(s, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < PS; ++q) part[((size_t)cell * N + i) * PS + q] = s[q];
+}
+
+// Q <- Q - C^T (C C^T)^-1 C Q on x-plane i (src/conserve.c:207-264 with one
+// species); every workgroup of a cell combines the cell's N partials and
+// solves the 5 x 5 system itself.
+__global__ void __launch_bounds__(XT) k_proj_apply(int N, const double *__restrict__ v, const double *__restrict__ w,
+                                                   double pm, ProjArgs pa, double *__restrict__ Q,
+                                                   const double *__restrict__ part) {
+  __shared__ double lam[PS];
+  const int i = blockIdx.x, cell = blockIdx.y;
+  const int n2 = N * N;
+  if (threadIdx.x == 0) {
+    double b[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int x = 0; x < N; ++x)
+      for (int q = 0; q < PS; ++q) b[q] += part[((size_t)cell * N + x) * PS + q];
+    for (int k = 0; k < PS - 1; ++k) {
+      const int p = pa.pivot[k];
+      if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
+      for (int r = k + 1; r < PS; ++r) b[r] -= pa.LU[r * PS + k] * b[k];
+    }
+    b[PS - 1] = b[PS - 1] / pa.LU[(PS - 1) * PS + PS - 1];
+    for (int r = PS - 2; r >= 0; --r) {
+      double sum = 0.0;
+      for (int c = r + 1; c < PS; ++c) sum += pa.LU[r * PS + c] * b[c];
+      b[r] = 1.0 / pa.LU[r * PS + r] * (b[r] - sum);
+    }
+    for (int q = 0; q < PS; ++q) lam[q] = b[q];
+  }
+  __syncthreads();
+  double *qp = Q + (size_t)cell * n2 * N + (size_t)i * n2;
+  const double vi = v[i], wi = w[i] * pm;
+  for (int e = threadIdx.x; e < n2; e += XT) {
+    const int j = e / N, k = e - j * N;
+    const double p = wi * w[j] * w[k];
+    const double e2 = 0.5 * (vi * vi + v[j] * v[j] + v[k] * v[k]);
+    qp[e] -= p * lam[0] + p * vi * lam[1] + p * v[j] * lam[2] + p * v[k] * lam[3] + p * e2 * lam[4];
+  }
+}
+
+// FFT-capable grids (N = 2^a 3^b with both LDS buffers of a plane in 64 KiB).
+constexpr int kFftN[] = {6, 8, 9, 12, 16, 18, 24, 32, 36, 48};
+
+template <int N>
+void launch_fft_fwd(const spbte_grid_dev *g, const XformJobs &J, int njobs, int ncell, int out_fp32, void *scratch,
+                    hipStream_t s) {
+  const double2 *tf = (const double2 *)g->d_fft;  // forward p, q, tw
+  hipLaunchKernelGGL(k_fft_a_real<N>, dim3(N, ncell, njobs), dim3(XT), 0, s, J, ncell, tf, (const double *)g->d_v,
+                     (double2 *)scratch, -1.0);
+  if (out_fp32)
+    hipLaunchKernelGGL((k_fft_b<N, 1>), dim3(N, ncell, njobs), dim3(XT), 0, s, ncell, tf, g->pre_fwd,
+                       (const double2 *)scratch, J, nullptr, -1.0);
+  else
+    hipLaunchKernelGGL((k_fft_b<N, 0>), dim3(N, ncell, njobs), dim3(XT), 0, s, ncell, tf, g->pre_fwd,
+                       (const double2 *)scratch, J, nullptr, -1.0);
+}
+
+// Complex cube transform (invert 0/1, OUT 0 = complex, 2 = real part).
+template <int N>
+void launch_fft_cplx(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert, int real_out,
+                     void *scratch, hipStream_t s) {
+  const double2 *t = (const double2 *)g->d_fft + (invert ? 3 * N : 0);
+  const double dsgn = invert ? 1.0 : -1.0, pre = invert ? g->pre_inv : g->pre_fwd;
+  XformJobs J = {};
+  hipLaunchKernelGGL(k_fft_a_cplx<N>, dim3(N, ncell, 1), dim3(XT), 0, s, (const double2 *)in, ncell, t,
+                     (double2 *)scratch, dsgn);
+  if (real_out)
+    hipLaunchKernelGGL((k_fft_b<N, 2>), dim3(N, ncell, 1), dim3(XT), 0, s, ncell, t, pre, (const double2 *)scratch, J,
+                       out, dsgn);
+  else
+    hipLaunchKernelGGL((k_fft_b<N, 0>), dim3(N, ncell, 1), dim3(XT), 0, s, ncell, t, pre, (const double2 *)scratch, J,
+                       out, dsgn);
+}
+
+// Runs fn<N>() for the grid's N if it has FFT kernels; false otherwise.
+template <typename F, int I = 0>
+bool with_fft_n(int N, F &&fn) {
+  if constexpr (I < (int)(sizeof kFftN / sizeof kFftN[0])) {
+    if (N == kFftN[I]) {
+      fn(std::integral_constant<int, kFftN[I]>{});
+      return true;
+    }
+    return with_fft_n<F, I + 1>(N, static_cast<F &&>(fn));
+  } else {
+    return false;
+  }
+}
+
 }  // namespace
 
-extern "C" int spbte_launch_moments(const spbte_grid_dev *g, const double *f, double *mom, int ncell, hipStream_t s) {
-  hipLaunchKernelGGL(k_moments, dim3(ncell), dim3(XT), 0, s, g->N, f, (const double *)g->d_v, (const double *)g->d_w, g->hv,
+extern "C" int spbte_launch_moments(const spbte_grid_dev *g, const double *f, double *mom, int ncell, double *part,
+                                    hipStream_t s) {
+  if (!part) {
+    hipLaunchKernelGGL(k_moments, dim3(ncell), dim3(XT), 0, s, g->N, f, (const double *)g->d_v, (const double *)g->d_w,
+                       g->hv, g->mass, g->kb, mom);
+    return (int)hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_moments_part, dim3(g->N, ncell), dim3(XT), 0, s, g->N, f, (const double *)g->d_v,
+                     (const double *)g->d_w, part);
+  hipLaunchKernelGGL(k_moments_fin, dim3((ncell + 63) / 64), dim3(64), 0, s, g->N, ncell, (const double *)part, g->hv,
                      g->mass, g->kb, mom);
   return (int)hipGetLastError();
 }
@@ -279,6 +659,10 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
   if (njobs < 1 || njobs > 4) return (int)hipErrorInvalidValue;
   XformJobs J = {};
   for (int q = 0; q < njobs; ++q) J.j[q] = jobs[q];
+  if (g->d_fft && with_fft_n(g->N, [&](auto n) {
+        launch_fft_fwd<decltype(n)::value>(g, J, njobs, ncell, out_fp32, scratch, s);
+      }))
+    return (int)hipGetLastError();
   const int N = g->N;
   const size_t lds_a = 2 * (size_t)N * N * sizeof(double2), lds_b = (size_t)N * N * sizeof(double2);
   hipLaunchKernelGGL(k_xform_a_real, dim3(N, ncell, njobs), dim3(XT), lds_a, s, N, J, ncell,
@@ -294,6 +678,10 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
 
 extern "C" int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert,
                                               void *scratch, hipStream_t s) {
+  if (g->d_fft && with_fft_n(g->N, [&](auto n) {
+        launch_fft_cplx<decltype(n)::value>(g, in, out, ncell, invert, 0, scratch, s);
+      }))
+    return (int)hipGetLastError();
   const int N = g->N;
   const size_t lds_a = 2 * (size_t)N * N * sizeof(double2), lds_b = (size_t)N * N * sizeof(double2);
   const double2 *M = (const double2 *)(invert ? g->d_mat_inv : g->d_mat_fwd);
@@ -307,6 +695,10 @@ extern "C" int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const voi
 
 extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, double *q, int ncell, void *scratch,
                                            hipStream_t s) {
+  if (g->d_fft && with_fft_n(g->N, [&](auto n) {
+        launch_fft_cplx<decltype(n)::value>(g, Qh, q, ncell, 1, 1, scratch, s);
+      }))
+    return (int)hipGetLastError();
   const int N = g->N;
   const size_t lds_a = 2 * (size_t)N * N * sizeof(double2), lds_b = (size_t)N * N * sizeof(double2);
   XformJobs J = {};
@@ -318,9 +710,19 @@ extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *
 }
 
 extern "C" int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
-                                    double *Q, int ncell, hipStream_t s) {
+                                    double *Q, int ncell, double *part, hipStream_t s) {
   const size_t n3 = (size_t)g->N * g->N * g->N;
-  return spbte_launch_project_strided(g, Ns, masses, LU, pivot, Q, ncell, (size_t)Ns * Ns * n3, n3, s);
+  if (Ns != 1 || !part) return spbte_launch_project_strided(g, Ns, masses, LU, pivot, Q, ncell, (size_t)Ns * Ns * n3, n3, s);
+  ProjArgs pa = {};
+  for (int q = 0; q < PS * PS; ++q) pa.LU[q] = LU[q];
+  for (int q = 0; q < PS; ++q) pa.pivot[q] = pivot[q];
+  pa.masses[0] = masses[0];
+  const double pm = g->hv * g->hv * g->hv * masses[0];
+  hipLaunchKernelGGL(k_proj_part, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v, (const double *)g->d_w,
+                     pm, (const double *)Q, part);
+  hipLaunchKernelGGL(k_proj_apply, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
+                     (const double *)g->d_w, pm, pa, Q, (const double *)part);
+  return (int)hipGetLastError();
 }
 
 // Groups at Q + grp * gstride, the Ns^2 pairs of a group pstride apart
@@ -339,4 +741,10 @@ extern "C" int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, con
   hipLaunchKernelGGL(k_project, dim3(ncell), dim3(XT), 0, s, g->N, Ns, (const double *)g->d_v, (const double *)g->d_w, g->hv,
                      pa, Q, gstride, pstride);
   return (int)hipGetLastError();
+}
+
+// 1 if the transform of an N grid runs as FFTs (the grid then needs the factor
+// tables of spbte_grid_dev.d_fft).
+extern "C" int spbte_fft_supported(int N) {
+  return with_fft_n(N, [](auto) {}) ? 1 : 0;
 }

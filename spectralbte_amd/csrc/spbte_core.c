@@ -81,6 +81,7 @@ struct spbte_op {
   void *d_xs;        /* transform scratch: MAXJOBS * cap * n3 double2 */
   void *d_FG;        /* MAXJOBS * cap * n3 double2 (FP32 forms use half) */
   void *d_Qh;        /* cap * n3 double2 */
+  double *d_part;    /* 5 * N * cap: per-plane partial sums of the moments and the projection */
   /* tiles pinned per (form, nprod-1); tr == 0 -> automatic */
   spbte_tile pin[NFORMS][2];
   /* autotuned tiles per (form, nprod-1, ncell bucket) */
@@ -155,6 +156,26 @@ static void build_matrix(int N, const double *a, double delta, double Lstart, do
       M[2 * (k * N + j)] = (double)(cosl(ph) * w);
       M[2 * (k * N + j) + 1] = (double)(sinl(ph) * w);
     }
+}
+
+/* FFT factors of build_matrix's matrix, M = diag(q) DFT diag(p):
+ * p[j] = exp(i s j L_start delta) w_j, q[k] = exp(i s L_end a_k), and the DFT
+ * twiddles tw[m] = exp(i dsgn 2 pi m / N), in long double. */
+static void build_fft_factors(int N, const double *a, double delta, double Lstart, double Lend, double sign, double dsgn,
+                              double *out /* 2 * 3N: p, q, tw */) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int j = 0; j < N; j++) {
+    const double w = (j == 0 || j == N - 1) ? 0.5 : 1.0;
+    const long double ph = (long double)sign * j * Lstart * delta;
+    const long double qh = (long double)sign * Lend * a[j];
+    const long double th = (long double)dsgn * 2.0L * pi * (long double)j / N;
+    out[2 * j] = (double)(cosl(ph) * w);
+    out[2 * j + 1] = (double)(sinl(ph) * w);
+    out[2 * (N + j)] = (double)cosl(qh);
+    out[2 * (N + j) + 1] = (double)sinl(qh);
+    out[2 * (2 * N + j)] = (double)cosl(th);
+    out[2 * (2 * N + j) + 1] = (double)sinl(th);
+  }
 }
 
 /* ---------------------------------------------- LU (src/conserve.c:89-168) */
@@ -293,6 +314,14 @@ int spbte_op_create(spbte_op **out, int N, double L_v, const double *v, const do
     rc = fail(SPBTE_ERR_HIP, "device setup failed: %s", hipGetErrorString(hipGetLastError()));
   free(Mf);
   free(Mi);
+  if (rc == SPBTE_OK && spbte_fft_supported(N)) {
+    double ff[2 * 6 * SPBTE_MAX_N];
+    build_fft_factors(N, eta, g->hv, Leta, L_v, 1.0, -1.0, ff);
+    build_fft_factors(N, v, g->heta, L_v, Leta, -1.0, 1.0, ff + 6 * N);
+    if (hipMalloc(&g->d_fft, sizeof(double) * 12 * N) != hipSuccess ||
+        hipMemcpy(g->d_fft, ff, sizeof(double) * 12 * N, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(SPBTE_ERR_HIP, "device setup failed: %s", hipGetErrorString(hipGetLastError()));
+  }
   if (rc != SPBTE_OK) {
     spbte_op_destroy(op);
     return rc;
@@ -341,7 +370,9 @@ static void free_scratch(spbte_op *op) {
   if (op->d_xs) hipFree(op->d_xs);
   if (op->d_FG) hipFree(op->d_FG);
   if (op->d_Qh) hipFree(op->d_Qh);
+  if (op->d_part) hipFree(op->d_part);
   op->d_mom = NULL;
+  op->d_part = NULL;
   op->d_xs = op->d_FG = op->d_Qh = NULL;
   op->cap = 0;
 }
@@ -356,6 +387,7 @@ void spbte_op_destroy(spbte_op *op) {
   if (op->g.d_mat_inv) hipFree(op->g.d_mat_inv);
   if (op->g.d_v) hipFree(op->g.d_v);
   if (op->g.d_w) hipFree(op->g.d_w);
+  if (op->g.d_fft) hipFree(op->g.d_fft);
   if (op->s) hipStreamDestroy(op->s);
   free(op->v);
   free(op->eta);
@@ -383,6 +415,7 @@ static int ensure_scratch(spbte_op *op, int ncell) {
   HIPCHK(hipMalloc(&op->d_xs, 16 * n * MAXJOBS));
   HIPCHK(hipMalloc(&op->d_FG, 16 * n * MAXJOBS));
   HIPCHK(hipMalloc(&op->d_Qh, 16 * n));
+  HIPCHK(hipMalloc((void **)&op->d_part, sizeof(double) * 5 * (size_t)op->N * ncell));
   op->cap = ncell;
   op->gen++;
   return SPBTE_OK;
@@ -394,6 +427,7 @@ static int ensure_scratch(spbte_op *op, int ncell) {
  * must be resident and Qp must already hold qp_bytes. */
 typedef struct {
   double *mom; /* 2 * ncell * SPBTE_MOM_STRIDE */
+  double *part; /* 5 * N * ncell, per-plane partial sums */
   void *xs;    /* transform scratch */
   void *FG;    /* transformed operands */
   void *Qh;    /* Qhat, ncell * N^3 double2 */
@@ -403,7 +437,7 @@ typedef struct {
 } scratch_set;
 
 static scratch_set own_scratch(const spbte_op *op) {
-  scratch_set ss = {op->d_mom, op->d_xs, op->d_FG, op->d_Qh, NULL, 0, 0};
+  scratch_set ss = {op->d_mom, op->d_part, op->d_xs, op->d_FG, op->d_Qh, NULL, 0, 0};
   return ss;
 }
 
@@ -811,8 +845,8 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
     else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1]}; G0 = B[1]; }
   } else {
     /* Q = sum W [ T(g - M_f)[c] T(f)[s] + T(M_g)[c] T(f - M_f)[s] ] */
-    LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, op->s));
-    if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, op->s));
+    LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, ss->part, op->s));
+    if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, ss->part, op->s));
     jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};        /* F0 = T(f)        */
     jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1]};       /* F1 = T(f - M_f)  */
     F0 = B[0];
@@ -836,7 +870,7 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
   LCHK(spbte_launch_xform_inv_real(&op->g, ss->Qh, d_Q, ncell, ss->xs, op->s));
   if (flags & SPBTE_CONSERVE) {
     if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
-    LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
+    LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
   }
   return SPBTE_OK;
 }
@@ -885,25 +919,28 @@ int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, 
   const float *rs;
   int rc = (have && t.kind >= 1) ? weights_permuted(op, form, t.tr, 0, &W, &rs) : weights_form(op, form, 0, &W, &rs);
   if (rc) return rc;
-  /* workspace: moments | transformed operands | Qhat | transform scratch,
+  /* workspace: moments | partial sums | transformed operands | Qhat | transform scratch,
    * which also holds the split sum's partial sums (they are dead by the time
    * the inverse transform writes it) */
   const int nj = mp ? (self ? 3 : 4) : (self ? 1 : 2);
   const size_t n = (size_t)ncell * op->n3;
   const size_t b_mom = al256(sizeof(double) * 2 * (size_t)ncell * SPBTE_MOM_STRIDE);
+  const size_t b_part = al256(sizeof(double) * 5 * (size_t)op->N * ncell);
   const size_t b_fg = al256((size_t)nj * n * fg_bytes(form)), b_qh = al256(16 * n);
   const size_t b_xs = 16 * n * (size_t)nj, b_qp = (have && t.kind > 1) ? 16 * n * (size_t)t.kind : 0;
   const size_t b_x = al256(b_xs > b_qp ? b_xs : b_qp);
-  const size_t need = b_mom + b_fg + b_qh + b_x;
+  const size_t need = b_mom + b_part + b_fg + b_qh + b_x;
   if (need > op->nac_bytes)
     return fail(SPBTE_ERR_ARG, "a batch of %d cells at N=%d needs %zu bytes of workspace, more than the %zu of the "
                 "fixed workspace (SPBTE_NOALLOC_SCRATCH_BYTES)", ncell, op->N, need, op->nac_bytes);
   char *base = (char *)op->nac;
   scratch_set ss;
   ss.mom = (double *)base;
-  ss.FG = base + b_mom;
-  ss.Qh = base + b_mom + b_fg;
-  ss.xs = base + b_mom + b_fg + b_qh;
+  ss.part = (double *)(base + b_mom);
+  base += b_mom + b_part;
+  ss.FG = base;
+  ss.Qh = base + b_fg;
+  ss.xs = base + b_fg + b_qh;
   ss.Qp = ss.xs;
   ss.qp_bytes = b_x;
   ss.fixed = 1;
@@ -915,7 +952,8 @@ int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
   if (ncell > 0 && !d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
   if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
   if (ncell <= 0) return SPBTE_OK;
-  LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, op->s));
+  LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ncell <= op->cap ? op->d_part : NULL,
+                            op->s));
   return SPBTE_OK;
 }
 
@@ -939,7 +977,7 @@ int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups) {
   if (!op || ngroups < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ngroups == 0) return SPBTE_OK;
   if (!d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
-  LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, op->s));
+  LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, NULL, op->s));
   return SPBTE_OK;
 }
 

@@ -23,6 +23,8 @@ typedef struct {
   void *d_mat_inv;         /* double2[N*N]: per-axis inverse matrix */
   void *d_v;               /* double[N] */
   void *d_w;               /* double[N] trapezoid weights */
+  void *d_fft;             /* double2[6N]: forward p, q, twiddles, then inverse p, q, twiddles
+                              (FFT factors of the per-axis matrices), or NULL: matrix form */
 } spbte_grid_dev;
 
 /* Moments of one cell used by the Maxwellian split:
@@ -58,7 +60,12 @@ typedef struct {
 } spbte_tile;
 
 /* ----- launchers (k_*.hip); all return hipError_t of the launch ----- */
-int spbte_launch_moments(const spbte_grid_dev *g, const double *f, double *mom, int ncell, hipStream_t s);
+/* part: NULL (one workgroup per cell) or 5 * N * ncell doubles of scratch
+ * (one workgroup per x-plane and cell, then a combining pass); the same for
+ * the one-species projection. */
+int spbte_launch_moments(const spbte_grid_dev *g, const double *f, double *mom, int ncell, double *part,
+                         hipStream_t s);
+int spbte_fft_supported(int N);
 int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform_job *jobs, int njobs, int ncell, int out_fp32,
                            void *scratch, hipStream_t s);
 int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert,
@@ -79,7 +86,7 @@ int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const s
 int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,
                                  hipStream_t s);
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
-                         double *Q, int ncell, hipStream_t s);
+                         double *Q, int ncell, double *part, hipStream_t s);
 int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                                  double *Q, int ngroups, size_t gstride, size_t pstride, hipStream_t s);
 int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t n_rows, size_t n_cols, hipStream_t s);

@@ -224,7 +224,7 @@ int spbte_run_moments(spbte_run *r, double *out) {
   const size_t nm = (size_t)r->ns * r->ncell;
   for (int l = 0; l < r->ns; l++)
     LCHK(spbte_launch_moments(spbte_op_grid(r->op), interior(r, r->d_f + l * gblock(r)),
-                              r->d_mom + (size_t)l * r->ncell * SPBTE_MOM_STRIDE, r->ncell, s));
+                              r->d_mom + (size_t)l * r->ncell * SPBTE_MOM_STRIDE, r->ncell, NULL, s));
   HIPCHK(hipStreamSynchronize(s));
   double *m = malloc(sizeof(double) * nm * SPBTE_MOM_STRIDE);
   if (!m) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
