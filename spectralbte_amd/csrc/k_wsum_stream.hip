@@ -392,11 +392,13 @@ const SEntry kSTable[] = {
     S3(32, 4, 2, 1, 1), S3(32, 4, 2, 1, 2), S3(32, 4, 1, 1, 1), S3(32, 4, 1, 2, 1),
     // N = 24 (rows per workgroup must divide 24^3): TR=3 x 3 waves = one x-plane,
     // TR=6 x 3 waves = 2 planes, TR=4 x 2 waves and TR=8 x 1 wave straddle planes
-    S3(24, 3, 3, 1, 1), S3(24, 3, 3, 2, 1), S3(24, 3, 3, 1, 2), S3(24, 6, 3, 1, 1), S3(24, 6, 3, 1, 2),
-    S3(24, 4, 2, 1, 1), S3(24, 4, 2, 2, 1), S3(24, 4, 2, 1, 2), S3(24, 8, 1, 1, 1), S3(24, 8, 1, 1, 2),
-    S3(24, 4, 9, 1, 2),
-    // N = 16: TR=4 (1 wave = 1 x-plane)
+    S4(24, 3, 3, 1, 1), S4(24, 3, 3, 2, 1), S4(24, 3, 3, 1, 2), S3(24, 6, 3, 1, 1), S4(24, 6, 3, 1, 2),
+    S4(24, 4, 2, 1, 1), S4(24, 4, 2, 2, 1), S4(24, 4, 2, 1, 2), S3(24, 8, 1, 1, 1), S4(24, 8, 1, 1, 2),
+    S4(24, 4, 9, 1, 2), S(24, 3, 3, 2, 2, 3),
+    // N = 16: TR=4 (1 wave = 1 x-plane); TR=2 and TR=1 (half and quarter planes)
+    // give one cell more waves
     S3(16, 4, 1, 1, 1), S4(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S4(16, 4, 1, 2, 2),
+    S3(16, 2, 1, 1, 1), S3(16, 2, 1, 1, 2), S3(16, 1, 1, 1, 1), S3(16, 1, 1, 1, 2),
 };
 #undef S4
 #undef S3

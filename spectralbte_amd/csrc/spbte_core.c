@@ -703,30 +703,42 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
 
 /* Streaming kernel defaults (measured on 256-512 cells, MI355X): FP64 weights
  * favour 8 rows per lane for the two-product sum, FP32 weights 4 rows and 4
- * waves.  The FP32-product form has streaming tiles at N=16 and 32 only (the
- * tiled kernel serves it elsewhere: unsupported candidates are skipped).
+ * waves; the FP32-product maxPreserve sum at N=24 takes 3 rows x 3 waves with
+ * two problems per lane (31.7 ms for 512 problems against 44.2 ms on the tiled
+ * kernel).  Unsupported candidates are skipped.
  * Batches too small to give 2048 workgroups split the summed x-planes into up
- * to 8 slices. */
+ * to 8 slices (16 at N=16, where one to four cells take one row per lane and
+ * one x-plane per slice: a single cell's FP64 evaluation then streams its
+ * table in 48 us instead of 99). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}}, s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
+  static const spbte_tile s24ma[] = {{3, 3, 2, 1}, {3, 3, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
+  static const spbte_tile s16one[] = {{1, 1, 1, 1}};
   const spbte_tile *c = NULL;
   int nc = 0;
   if (N == 32) {
     if (nprod == 1) { c = s32; nc = 2; }
     else if (form == SPBTE_FORM_FP64) { c = s32m64; nc = 1; }
     else { c = s32m; nc = 1; }
-  } else if (N == 24) { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
-  else if (N == 16) { c = nprod == 2 ? s16m : s16; nc = 2; }
+  } else if (N == 24) {
+    if (nprod == 2 && form == SPBTE_FORM_FP32A) { c = s24ma; nc = 2; }
+    else { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
+  }
+  else if (N == 16) {
+    if (ncell <= 4 && spbte_stream_tile_supported(N, form, nprod, 1, 1, 1)) { c = s16one; nc = 1; }
+    else { c = nprod == 2 ? s16m : s16; nc = 2; }
+  }
   else return 0;
+  const int kmax = N <= 16 ? N : 8;
   for (int i = 0; i < nc; i++) {
     if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
     if (c[i].tb > ncell && i + 1 < nc) continue;
     *t = c[i];
     const long wgs = stream_wgs(N, ncell, t);
     int k = 1;
-    while (k < 8 && wgs * k < 2048 && N % (2 * k) == 0) k *= 2;
+    while (k < kmax && wgs * k < 2048 && N % (2 * k) == 0) k *= 2;
     t->kind = k;
     return 1;
   }
@@ -1072,7 +1084,9 @@ static const spbte_tile k_cand[] = {
     {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1},
     /* streaming with the summed x-planes split in 2-8 slices, for small batches */
     {4, 4, 1, 2}, {4, 4, 1, 4}, {4, 4, 1, 8}, {4, 4, 2, 2}, {4, 4, 2, 4}, {4, 4, 2, 8}, {4, 2, 1, 4}, {4, 2, 1, 8},
-    {4, 2, 2, 4}, {8, 2, 1, 4}, {8, 2, 1, 8}, {4, 2, 1, 2}, {4, 9, 1, 2}, {4, 9, 1, 4}};
+    {4, 2, 2, 4}, {8, 2, 1, 4}, {8, 2, 1, 8}, {4, 2, 1, 2}, {4, 9, 1, 2}, {4, 9, 1, 4},
+    /* one or a few cells at N=16: narrower rows, up to one x-plane per slice */
+    {4, 1, 1, 8}, {4, 1, 1, 16}, {2, 1, 1, 8}, {2, 1, 1, 16}, {1, 1, 1, 8}, {1, 1, 1, 16}, {8, 2, 1, 16}};
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");

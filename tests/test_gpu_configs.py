@@ -104,6 +104,35 @@ def test_n16_fp32a_two_cell_groups_nontemporal(spb, orc):
     op.close()
 
 
+def test_ensemble_n24_fp32a_against_oracle(spb, orc):
+    """The N=24 half of the ensemble (512 problems, L_v=6, hard spheres) in the
+    FP32-product form on the streaming kernel's N=24 tiles (3 rows x 3 waves and
+    4 rows x 2 waves, one and two problems per lane) and on the default tile, problems
+    at both ends of the batch against the oracle: within 1e-6."""
+    from spectralbte_amd import workloads
+    N, Lv = 24, 6.0
+    v, eta = workloads.grid_0d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    f = workloads.two_maxwellian_ensemble(N, Lv, 512, seed=0x5EED0024)
+    W = op.get_weights()
+    cons = orc.Conserver(N, v)
+    ref = {b: cons(orc.compute_q_maxpreserve(N, v, eta, W, f[b], f[b]).ravel()) for b in (0, 511)}
+    del W
+    for tile in ((1, 4, 2, 1), (1, 3, 3, 1), (1, 3, 3, 2), None):   # (kind, rows, waves, problems per lane)
+        if tile:
+            op.set_tile("fp32a", True, tile[1], tile[2], tile[3], kind=tile[0])
+        else:
+            op.set_tile("fp32a", True, 0, 0, 0)
+            assert op.get_tile(512, "fp32a", True)[0] >= 1   # the streaming kernel by default
+        q = op.compute_q(f, form="fp32a", maxpreserve=True, conserve=True)
+        for b, r in ref.items():
+            e = rel_l2(q[b], r)
+            print(f"tile {tile} problem {b}: fp32a {e:.2e}")
+            assert e < 1e-6, (tile, b, e)
+    op.close()
+
+
 SHOCK_CELLS = (5, 122, 128, 134, 250)   # upstream, three in the layer, downstream
 
 
