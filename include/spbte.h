@@ -330,24 +330,27 @@ int spbte_net_forward_features(spbte_net *net, const float *X, int B, float *Y);
 int spbte_net_grad_features(spbte_net *net, const float *X, const float *T, int B, float *grad, double *loss);
 /* Surrogate Q of ncell cells (device arrays), optionally projected. */
 int spbte_net_predict(spbte_net *net, const double *d_f, double *d_Q, int ncell, int conserve);
-/* Surrogate-assisted Q with a sampled full-spectral check: every check_every-th
- * cell from `phase` is also evaluated exactly (precision form `form`) and keeps
- * the exact Q; if the surrogate's error on any of them exceeds tol, all cells
- * fall back to the exact operator.  Q is projected in every case.  The error of
- * a checked cell is |Q_net - Q| / max(|Q|, floor_rel * max over checked cells
- * of |Q|): floor_rel = 0 is the plain relative L2 error, which equilibrium
- * cells (whose exact Q is only the discretisation residual) fail by
- * construction.  n_exact: cells evaluated exactly; max_err: largest sampled
- * error.  Synchronous. */
+/* Surrogate-assisted Q with a sampled full-spectral check, per group: the
+ * cells are taken in groups of check_every consecutive cells; one cell of each
+ * group (the one at `phase` within it, or its last cell if the group is
+ * shorter) is also evaluated exactly (precision form `form`), keeps the exact
+ * Q and judges its group: if the surrogate's error there exceeds tol, every
+ * cell of the group falls back to the exact operator, otherwise the group's
+ * other cells keep the surrogate's Q.  Q is projected in every case.  The
+ * error of a checked cell is |Q_net - Q| / max(|Q|, floor_rel * max over
+ * checked cells of |Q|): floor_rel = 0 is the plain relative L2 error, which
+ * equilibrium cells (whose exact Q is only the discretisation residual) fail
+ * by construction.  n_exact: cells evaluated exactly (checked cells
+ * included); max_err: largest checked error.  Synchronous. */
 int spbte_net_collide(spbte_net *net, const double *d_f, double *d_Q, int ncell, int form, int check_every, int phase,
                       double tol, double floor_rel, int *n_exact, double *max_err);
 /* Loss denominator floor: each cell's loss is |y - t|^2 / (|t|^2 + 2 Kb^3 rms_floor^2)
  * in the calibrated target units (default rms_floor 0.1). */
 int spbte_net_set_loss_floor(spbte_net *net, double rms_floor);
-/* Runs whose collision stages use spbte_net_collide (the checked cells rotate
- * by one each stage); the statistics count cells, exactly evaluated cells,
- * stages that fell back, and the largest sampled error.  Such runs do not use
- * graphs. */
+/* Runs whose collision stages use spbte_net_collide (the checked cell of each
+ * group rotates by one each stage); the statistics count cells, exactly
+ * evaluated cells, groups that fell back, and the largest sampled error.  Such
+ * runs do not use graphs. */
 int spbte_run_set_surrogate(spbte_run *run, spbte_net *net, int check_every, double tol, double floor_rel);
 int spbte_run_surrogate_stats(spbte_run *run, long *cells, long *exact, long *fallbacks, double *max_err);
 

@@ -636,3 +636,20 @@ extern "C" int spbte_net_cell_norm(long n, int ncell, const double *a, double *o
   hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, n, a, out);
   return (int)hipGetLastError();
 }
+
+// Cells by index: scatter == 0 copies src cell idx[b] to dst cell b (gather),
+// scatter != 0 copies src cell b to dst cell idx[b]; cells of n doubles.
+__global__ void __launch_bounds__(256) k_move_cells(long n, const int *__restrict__ idx, const double *__restrict__ src,
+                                                    double *__restrict__ dst, int scatter) {
+  const long b = blockIdx.y;
+  const long from = scatter ? b : idx[b], to = scatter ? idx[b] : b;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) dst[to * n + e] = src[from * n + e];
+}
+
+extern "C" int spbte_net_move_cells(long n, int ncell, const int *idx, const double *src, double *dst, int scatter,
+                                    hipStream_t s) {
+  if (ncell < 1) return (int)hipSuccess;
+  const long blocks = (n + 255) / 256 < 64 ? (n + 255) / 256 : 64;
+  hipLaunchKernelGGL(k_move_cells, dim3((unsigned)blocks, (unsigned)ncell), dim3(256), 0, s, n, idx, src, dst, scatter);
+  return (int)hipGetLastError();
+}

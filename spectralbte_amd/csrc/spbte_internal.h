@@ -142,6 +142,12 @@ int spbte_net_adam(long n, float *p, const float *g, float *m1, float *m2, float
 int spbte_net_axpy(long n, float a, const float *x, float *y, hipStream_t s);
 int spbte_net_cell_relerr(long n, int ncell, const double *a, const double *b, double *out, hipStream_t s);
 int spbte_net_cell_norm(long n, int ncell, const double *a, double *out, hipStream_t s);
+/* spbte_net_collide with the count of groups that fell back (spbte_net.c). */
+struct spbte_net;
+int spbte_net_collide_groups(struct spbte_net *n, const double *d_f, double *d_Q, int ncell, int form, int check_every,
+                             int phase, double tol, double floor_rel, int *n_exact, int *n_failed, double *max_err);
+int spbte_net_move_cells(long n, int ncell, const int *idx, const double *src, double *dst, int scatter,
+                         hipStream_t s);
 int spbte_net_relayout(long M, int nsamp, int width, const float *src, float *dst, int to_samples, const int *idx,
                        hipStream_t s);
 

@@ -74,8 +74,9 @@ struct spbte_net {
   double *mom;
   void *Fd, *Fm, *xs, *Qh;
   /* surrogate-assisted collisions: sampled cells */
-  int scap;
+  int scap, ccap;
   double *fs, *Qs, *Qn, *err;
+  int *d_cells;
   /* dataset (sample-major) */
   long ds_cap, ds_n;
   float *dsX, *dsT;
@@ -163,6 +164,7 @@ void spbte_net_destroy(spbte_net *n) {
   double *d[] = {n->fs, n->Qs, n->Qn, n->err};
   for (size_t i = 0; i < sizeof d / sizeof d[0]; i++)
     if (d[i]) hipFree(d[i]);
+  if (n->d_cells) hipFree(n->d_cells);
   free(n);
 }
 
@@ -575,49 +577,64 @@ int spbte_net_layout(spbte_net *n, long *offsets) {
   return k;
 }
 
-/* Surrogate-assisted collision operator for ncell cells.  Every check_every-th
- * cell (from `phase`) is evaluated exactly with the operator (form `form`);
- * those cells keep the exact Q.  If the surrogate's relative L2 error on any
- * of them exceeds tol, every cell falls back to the exact operator; otherwise
- * the other cells take the surrogate Q.  Q is projected in every case.
- * n_exact: cells evaluated exactly; max_err: largest sampled error. */
-int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, int form, int check_every, int phase,
-                      double tol, double floor_rel, int *n_exact, double *max_err) {
+/* Surrogate-assisted collision operator for ncell cells, checked per group.
+ * The cells are taken in groups of check_every consecutive cells.  One cell of
+ * every group -- the one at `phase` within the group, or the group's last cell
+ * if the group is shorter -- is evaluated exactly (form `form`), keeps the
+ * exact Q and judges its group: if the surrogate's error there exceeds tol,
+ * every cell of the group is evaluated exactly; otherwise the group's other
+ * cells keep the surrogate's Q.  Q is projected in every case.  n_exact: cells
+ * evaluated exactly (checked cells included); n_failed: groups that fell
+ * back; max_err: largest error over the checked cells. */
+int spbte_net_collide_groups(spbte_net *n, const double *d_f, double *d_Q, int ncell, int form, int check_every,
+                             int phase, double tol, double floor_rel, int *n_exact, int *n_failed, double *max_err) {
   if (!n || !d_f || !d_Q || ncell < 1 || check_every < 1) return spbte_fail(SPBTE_ERR_ARG, "bad arguments");
   const size_t n3 = (size_t)n->N * n->N * n->N;
   const int flags = SPBTE_SELF | SPBTE_CONSERVE | (n->maxpreserve ? SPBTE_MAXPRESERVE : 0);
-  phase = ((phase % check_every) + check_every) % check_every;
-  const int k = phase < ncell ? (ncell - 1 - phase) / check_every + 1 : 0;
+  const int ce = check_every;
+  phase = ((phase % ce) + ce) % ce;
+  const int k = (ncell + ce - 1) / ce; /* groups, one checked cell each */
   if (k > n->scap) {
     HIPCHK(hipStreamSynchronize(n->s));
     double **d[] = {&n->fs, &n->Qs, &n->Qn, &n->err};
     for (size_t i = 0; i < 4; i++)
       if (*d[i]) { hipFree(*d[i]); *d[i] = NULL; }
+    if (n->d_cells) { hipFree(n->d_cells); n->d_cells = NULL; }
     HIPCHK(hipMalloc((void **)&n->fs, sizeof(double) * n3 * k));
     HIPCHK(hipMalloc((void **)&n->Qs, sizeof(double) * n3 * k));
     HIPCHK(hipMalloc((void **)&n->Qn, sizeof(double) * n3 * k));
     HIPCHK(hipMalloc((void **)&n->err, sizeof(double) * k));
     n->scap = k;
   }
-  CHK(spbte_net_predict(n, d_f, d_Q, ncell, 1));
-  double worst = 0.0;
-  if (k > 0) {
-    for (int j = 0; j < k; j++) {
-      const size_t c = (size_t)phase + (size_t)j * check_every;
-      HIPCHK(hipMemcpyAsync(n->fs + j * n3, d_f + c * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
-      HIPCHK(hipMemcpyAsync(n->Qn + j * n3, d_Q + c * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
-    }
-    CHK(spbte_compute_q(n->op, n->fs, NULL, n->Qs, k, form, flags));
-    /* err_j = |Qn_j - Qs_j| / max(|Qs_j|, floor_rel * max_i |Qs_i|) */
-    double *he = malloc(sizeof(double) * 3 * k);
-    if (!he) return spbte_fail(SPBTE_ERR_ARG, "out of host memory");
-    int lrc = spbte_net_cell_relerr((long)n3, k, n->Qn, n->Qs, n->err, n->s);
-    hipError_t e = (hipError_t)lrc;
+  if (ncell > n->ccap) {
+    HIPCHK(hipStreamSynchronize(n->s));
+    if (n->d_cells) hipFree(n->d_cells);
+    n->d_cells = NULL;
+    HIPCHK(hipMalloc((void **)&n->d_cells, sizeof(int) * ncell));
+    n->ccap = ncell;
+  }
+  int *idx = malloc(sizeof(int) * (size_t)ncell);
+  double *he = malloc(sizeof(double) * 3 * (size_t)k);
+  if (!idx || !he) { free(idx); free(he); return spbte_fail(SPBTE_ERR_ARG, "out of host memory"); }
+  int rc = spbte_net_predict(n, d_f, d_Q, ncell, 1);
+  /* the checked cells: gather f and the surrogate's Q, evaluate exactly */
+  for (int j = 0; j < k; j++) {
+    const int len = (j + 1) * ce <= ncell ? ce : ncell - j * ce;
+    idx[j] = j * ce + (phase < len ? phase : len - 1);
+  }
+  hipError_t e = rc ? hipSuccess : hipMemcpyAsync(n->d_cells, idx, sizeof(int) * k, hipMemcpyHostToDevice, n->s);
+  if (!rc && e == hipSuccess) e = (hipError_t)spbte_net_move_cells((long)n3, k, n->d_cells, d_f, n->fs, 0, n->s);
+  if (!rc && e == hipSuccess) e = (hipError_t)spbte_net_move_cells((long)n3, k, n->d_cells, d_Q, n->Qn, 0, n->s);
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(n->s); /* idx is reused below */
+  if (!rc && e != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "gathering the checked cells: %s", hipGetErrorString(e));
+  if (!rc) rc = spbte_compute_q(n->op, n->fs, NULL, n->Qs, k, form, flags);
+  /* err_j = |Qn_j - Qs_j| / max(|Qs_j|, floor_rel * max_i |Qs_i|) */
+  if (!rc) {
+    e = (hipError_t)spbte_net_cell_relerr((long)n3, k, n->Qn, n->Qs, n->err, n->s);
     if (e == hipSuccess) e = hipMemcpyAsync(he, n->err, sizeof(double) * k, hipMemcpyDeviceToHost, n->s);
     if (e == hipSuccess) e = hipStreamSynchronize(n->s);
     if (e == hipSuccess && floor_rel > 0) {
-      lrc = spbte_net_cell_norm((long)n3, k, n->Qs, n->err, n->s);
-      e = (hipError_t)lrc;
+      e = (hipError_t)spbte_net_cell_norm((long)n3, k, n->Qs, n->err, n->s);
       if (e == hipSuccess) e = hipMemcpyAsync(he + k, n->err, sizeof(double) * k, hipMemcpyDeviceToHost, n->s);
       if (e == hipSuccess) e = hipStreamSynchronize(n->s);
       if (e == hipSuccess) e = (hipError_t)spbte_net_cell_norm((long)n3, k, n->Qn, n->err, n->s);
@@ -633,23 +650,49 @@ int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, i
         he[j] = fl > 0 ? dq / fl : he[j];
       }
     }
-    for (int j = 0; j < k && e == hipSuccess; j++)
+    if (e != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "error check failed: %s", hipGetErrorString(e));
+  }
+  double worst = 0.0;
+  int nf = 0, nb = 0; /* failed groups, cells to re-evaluate */
+  if (!rc) {
+    for (int j = 0; j < k; j++) {
       if (!(he[j] <= worst)) worst = he[j]; /* NaN counts as a failure */
-    free(he);
-    if (e != hipSuccess) return spbte_fail(SPBTE_ERR_HIP, "error check failed: %s", hipGetErrorString(e));
+      if (!(he[j] <= tol)) {
+        nf++;
+        const int end = (j + 1) * ce <= ncell ? (j + 1) * ce : ncell;
+        for (int c = j * ce; c < end; c++)
+          if (c != idx[j]) idx[k + nb++] = c; /* after the checked cells' indices */
+      }
+    }
   }
+  /* checked cells keep the exact Q */
+  if (!rc) {
+    e = (hipError_t)spbte_net_move_cells((long)n3, k, n->d_cells, n->Qs, d_Q, 1, n->s);
+    if (e != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "scattering the checked cells: %s", hipGetErrorString(e));
+  }
+  /* the other cells of failed groups: exact, in batches of at most k cells
+   * through the checked cells' buffers */
+  for (int b0 = 0; b0 < nb && !rc; b0 += k) {
+    const int m = nb - b0 < k ? nb - b0 : k;
+    e = hipMemcpyAsync(n->d_cells, idx + k + b0, sizeof(int) * m, hipMemcpyHostToDevice, n->s);
+    if (e == hipSuccess) e = (hipError_t)spbte_net_move_cells((long)n3, m, n->d_cells, d_f, n->fs, 0, n->s);
+    if (e != hipSuccess) { rc = spbte_fail(SPBTE_ERR_HIP, "gathering fallback cells: %s", hipGetErrorString(e)); break; }
+    rc = spbte_compute_q(n->op, n->fs, NULL, n->Qs, m, form, flags);
+    if (!rc) e = (hipError_t)spbte_net_move_cells((long)n3, m, n->d_cells, n->Qs, d_Q, 1, n->s);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(n->s); /* before d_cells is rewritten */
+    if (!rc && e != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "fallback cells: %s", hipGetErrorString(e));
+  }
+  if (!rc && hipStreamSynchronize(n->s) != hipSuccess) rc = spbte_fail(SPBTE_ERR_HIP, "surrogate step failed");
+  free(idx);
+  free(he);
+  if (rc) return rc;
+  if (n_exact) *n_exact = k + nb;
+  if (n_failed) *n_failed = nf;
   if (max_err) *max_err = worst;
-  if (!(worst <= tol)) {
-    CHK(spbte_compute_q(n->op, d_f, NULL, d_Q, ncell, form, flags));
-    HIPCHK(hipStreamSynchronize(n->s));
-    if (n_exact) *n_exact = ncell;
-    return SPBTE_OK;
-  }
-  for (int j = 0; j < k; j++) {
-    const size_t c = (size_t)phase + (size_t)j * check_every;
-    HIPCHK(hipMemcpyAsync(d_Q + c * n3, n->Qs + j * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, n->s));
-  }
-  HIPCHK(hipStreamSynchronize(n->s));
-  if (n_exact) *n_exact = k;
   return SPBTE_OK;
+}
+
+int spbte_net_collide(spbte_net *n, const double *d_f, double *d_Q, int ncell, int form, int check_every, int phase,
+                      double tol, double floor_rel, int *n_exact, double *max_err) {
+  return spbte_net_collide_groups(n, d_f, d_Q, ncell, form, check_every, phase, tol, floor_rel, n_exact, NULL, max_err);
 }

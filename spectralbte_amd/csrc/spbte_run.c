@@ -321,14 +321,14 @@ static int collide(spbte_run *r, const double *d_in, size_t stride) {
     return rc ? rc : spbte_conserve_pairs(op, r->d_Q, r->ncell);
   }
   if (!r->net) return spbte_compute_q(r->op, d_in, NULL, r->d_Q, r->ncell, r->form, r->flags);
-  int n_exact = 0;
+  int n_exact = 0, n_failed = 0;
   double err = 0.0;
-  CHK(spbte_net_collide(r->net, d_in, r->d_Q, r->ncell, r->form, r->check_every, r->phase, r->tol, r->floor_rel,
-                        &n_exact, &err));
+  CHK(spbte_net_collide_groups(r->net, d_in, r->d_Q, r->ncell, r->form, r->check_every, r->phase, r->tol,
+                               r->floor_rel, &n_exact, &n_failed, &err));
   r->phase = (r->phase + 1) % r->check_every;
   r->s_cells += r->ncell;
   r->s_exact += n_exact;
-  if (n_exact == r->ncell && r->check_every > 1) r->s_fallbacks++;
+  r->s_fallbacks += n_failed;
   if (!(err <= r->s_max_err)) r->s_max_err = err;
   return SPBTE_OK;
 }

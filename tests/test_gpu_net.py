@@ -80,3 +80,63 @@ def test_net_trains_and_predicts_conserving(spb, orc):
     assert np.abs(Q @ Cm.T).max() < 1e-12 * (np.abs(Q) @ np.abs(Cm.T)).max()
     net.close()
     op.close()
+
+
+def test_net_collide_per_group_fallback(spb, orc):
+    """spbte_net_collide on 64 shock-profile cells (N=16, L_v=8, hard spheres) in
+    groups of 4: every checked cell keeps the exact Q; a group whose checked
+    cell's error exceeds tol is evaluated exactly, cell by cell equal to the
+    exact operator; the other groups keep the network's prediction bit for bit;
+    the count of exact cells is the checked cells plus the fallback groups'
+    other cells.  tol = 0 sends every group to the exact operator, a huge tol
+    none.  The fraction of cells the network serves at tol 1e-3 (error floored
+    at 0.1 of the largest checked |Q|) is printed."""
+    from spectralbte_amd import workloads
+    N, Lv, ncell, ce = 16, 8.0, 64, 4
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    f = np.concatenate([workloads.shock_profile(N, Lv, 32, mach=m, blend_cells=8.0) for m in (1.2, 1.4)])
+    d_f = spb.DeviceArray.from_host(f)
+    d_Q = spb.DeviceArray(f.shape)
+    net = spb.Net(op, 12, width=32, rank=32, layers=2, maxpreserve=False, seed=5)
+    net.calibrate(d_f.ptr, d_Q.ptr, ncell)
+    net.reserve(ncell)
+    net.add(d_f.ptr, d_Q.ptr, ncell)
+    net.train(200, 16, 2e-3)
+    Qe = op.compute_q(f, conserve=True)
+    Qn = net.predict(f, conserve=True)
+    for phase in (0, 3):
+        for tol, floor_rel in ((1e-3, 0.1), (0.0, 0.1), (1e30, 0.0)):
+            n_exact, worst = net.collide_dev(d_f.ptr, d_Q.ptr, ncell, "fp64", check_every=ce, phase=phase, tol=tol,
+                                             floor_rel=floor_rel)
+            op.sync()
+            Q = d_Q.download()
+            checked = [g * ce + phase for g in range(ncell // ce)]
+            qn = np.linalg.norm(Qe[checked], axis=1)
+            fl = floor_rel * qn.max()
+            errs = np.linalg.norm(Qn[checked] - Qe[checked], axis=1) / np.maximum(qn, fl)
+            assert abs(worst - errs.max()) <= 1e-9 * max(errs.max(), 1e-300), (worst, errs.max())
+            exact_cells = 0
+            for g, c in enumerate(checked):
+                cells = range(g * ce, (g + 1) * ce)
+                assert rel_l2(Q[c], Qe[c]) < 1e-12
+                if errs[g] > tol:
+                    exact_cells += ce
+                    for x in cells:
+                        assert rel_l2(Q[x], Qe[x]) < 1e-12, (tol, g, x)
+                else:
+                    exact_cells += 1
+                    for x in cells:
+                        if x != c:
+                            assert np.array_equal(Q[x], Qn[x]), (tol, g, x)
+            assert n_exact == exact_cells, (tol, n_exact, exact_cells)
+            if tol == 0.0:
+                assert n_exact == ncell
+            if tol > 1e29:
+                assert n_exact == ncell // ce
+            if tol == 1e-3:
+                print(f"phase {phase}: the network serves {ncell - n_exact} of {ncell} cells at tol 1e-3 "
+                      f"(floor 0.1); largest checked error {worst:.3g}")
+    net.close()
+    op.close()
