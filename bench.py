@@ -184,8 +184,16 @@ def main():
     alg_ceiling = ceil_tf * flops_per_pair(wl.mp) / (2.0 * instr)
     log(f"issue ceiling ({'FP32' if a.form == 'fp32a' else 'FP64'} FMA chains): {ceil_tf:.1f} TFLOP/s at {ceil_ghz:.2f} GHz;"
         f" the kernel's instruction mix can reach {alg_ceiling:.1f}, it reaches {achieved:.1f}")
-    roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": tsrc,
+    # one cell group per launch (the plain form) reads the table once per cell:
+    # below the ridge of peak flops over HBM bandwidth the launch is HBM-bound
+    hbm_bound = flops / hbm_min < peak * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if hbm_bound:
+        main = {"bound": "hbm", "achieved": round(hbm_min / (mk * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm_min / (mk * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    else:
+        main = {"bound": "valu", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4)}
+    roof = {**main, "traffic": traffic, "traffic_source": tsrc, "valu_tflops": round(achieved, 2),
             "kernel": "k_wsum_stream" if tile[0] >= 1 else "k_wsum", "kernel_ms": round(mk, 4),
             "alg_flops_per_launch": flops,
             "issue_ceiling_measured_tflops": round(ceil_tf, 2), "issue_ceiling_clock_ghz": round(ceil_ghz, 3),
@@ -526,12 +534,17 @@ def surrogate_run(spb, form="fp64", ncell=4096, train_cells=2048, steps=2500, ba
     t0 = time.perf_counter()
     sg.build_dataset(spb, op, net, ftr)
     t_data = time.perf_counter() - t0
+    log(f"surrogate: dataset of {train_cells} cells in {t_data:.1f} s; training")
     t0 = time.perf_counter()
     k1 = int(0.7 * steps)  # learning rate 1e-3, then 3e-4 for the last 30%
-    losses = np.concatenate([net.train(k1, batch, 1e-3, seed=1), net.train(steps - k1, batch, 3e-4, seed=2)])
+    losses = [net.train(k1, batch, 1e-3, seed=1)]
+    log(f"surrogate: {k1} Adam steps, loss {losses[-1][-1]:.3g}")
+    losses.append(net.train(steps - k1, batch, 3e-4, seed=2))
+    losses = np.concatenate(losses)
     t_train = time.perf_counter() - t0
     rel, flo, qn = sg.judged_errors(spb, op, net, fte)
     sig = qn >= 1e-2
+    log(f"surrogate: trained in {t_train:.1f} s; judging and timing on {ncell} cells")
     f = workloads.shock_profile(N, Lv, ncell, mach=1.2, blend_cells=20.0 * ncell / 256)
     t_pred = sg.time_predict(spb, op, net, f)
     fl = sg.forward_flops(Kb ** 3, C, r, L) * ncell
@@ -539,8 +552,7 @@ def surrogate_run(spb, form="fp64", ncell=4096, train_cells=2048, steps=2500, ba
     d_Q = spb.DeviceArray(f.shape)
     flags = spb.SELF | spb.CONSERVE
     exact = {}
-    for fm in (form, "fp32a"):
-        op.autotune(ncell, fm, False)
+    for fm in (form, "fp32a"):  # default tiles: autotuning 4096 cells would take minutes
         op.compute_q_dev(d_f.ptr, None, d_Q.ptr, ncell, fm, flags)
         op.sync()
         t0 = time.perf_counter()
@@ -549,6 +561,7 @@ def surrogate_run(spb, form="fp64", ncell=4096, train_cells=2048, steps=2500, ba
         exact[fm] = time.perf_counter() - t0
         if fm == form:
             Qe = d_Q.download()
+        log(f"surrogate: exact {fm} {ncell / exact[fm]:.0f} Q/s")
     net.collide_dev(d_f.ptr, d_Q.ptr, ncell, form, check_every=check_every, phase=0, tol=1e-3, floor_rel=0.1)
     op.sync()
     t0 = time.perf_counter()

@@ -21,7 +21,7 @@
 #include "spbte.h"
 #include "spbte_internal.h"
 
-#define CACHE_SLOTS 8
+#define CACHE_SLOTS (SPBTE_MAX_NS * SPBTE_MAX_NS)
 
 static spbte_op *ops[CACHE_SLOTS]; /* one operator (grid + table) per weight table */
 static double **keys[CACHE_SLOTS];
@@ -106,10 +106,33 @@ static spbte_op *op_for(double **conv_weights) {
     if (ages[i] < ages[oldest]) oldest = i;
   }
   if (slot < 0) {
-    slot = oldest;
-    if (ops[slot]) spbte_op_destroy(ops[slot]);
-    ops[slot] = NULL;
-    keys[slot] = NULL;
+    /* a free slot, unless HBM cannot hold one more operator (its FP64 table,
+     * the copy the streaming kernel reads and scratch): then evict the least
+     * recently used tables until it can */
+    const size_t n3 = (size_t)N * N * N, need = 2 * sizeof(double) * n3 * n3 + ((size_t)256 << 20);
+    slot = -1;
+    for (int i = 0; i < CACHE_SLOTS && slot < 0; i++)
+      if (!ops[i]) slot = i;
+    for (;;) {
+      size_t free_b = 0, total_b = 0;
+      int used = 0;
+      for (int i = 0; i < CACHE_SLOTS; i++) used += ops[i] != NULL;
+      if (slot >= 0 && (used == 0 || (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b >= need))) break;
+      int lru = -1;
+      for (int i = 0; i < CACHE_SLOTS; i++)
+        if (ops[i] && (lru < 0 || ages[i] < ages[lru])) lru = i;
+      if (lru < 0) break;
+      static int warned;
+      if (!warned)
+        fprintf(stderr, "spectralbte_amd: device memory holds %d weight tables of N=%d; evicting the least recently "
+                "used one (each eviction re-uploads a table on its next use)\n", used, N);
+      warned = 1;
+      spbte_op_destroy(ops[lru]);
+      ops[lru] = NULL;
+      keys[lru] = NULL;
+      if (slot < 0) slot = lru;
+    }
+    if (slot < 0) slot = oldest;
     if (spbte_op_create(&ops[slot], N, L_v, v, eta) != SPBTE_OK) die("ComputeQ");
     if (spbte_op_set_species(ops[slot], 1, &g_mass) != SPBTE_OK) die("ComputeQ");
     if (spbte_op_set_weights_rows(ops[slot], (const double *const *)conv_weights) != SPBTE_OK) die("ComputeQ");

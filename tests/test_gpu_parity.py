@@ -37,7 +37,10 @@ def _cmoments(N, v, Q):
 
 
 # ------------------------------------------------------------------ transforms
-@pytest.mark.parametrize("N,Lv,kind", [(8, 5.0, "0d"), (16, 5.0, "0d"), (24, 6.0, "0d"), (32, 8.0, "1d"), (12, 7.0, "1d")])
+# Grids of 2^a 3^b points run as LDS FFTs (radix 4, 2, 3: N=6 is 2x3, 9 is 3x3,
+# 18 is 2x3x3, 48 is 4x4x3); N=10 keeps the per-axis matrix form.
+@pytest.mark.parametrize("N,Lv,kind", [(8, 5.0, "0d"), (16, 5.0, "0d"), (24, 6.0, "0d"), (32, 8.0, "1d"), (12, 7.0, "1d"),
+                                       (6, 4.0, "0d"), (9, 5.0, "1d"), (18, 6.0, "0d"), (48, 8.0, "1d"), (10, 5.0, "0d")])
 @pytest.mark.parametrize("invert", [0, 1])
 def test_transform(spb, orc, N, Lv, kind, invert):
     v, eta = _grid(orc, N, Lv, kind)

@@ -124,3 +124,24 @@ def test_boltz_two_species_files(spb, orc, tmp_path):
                        [orc.moments_row_0d(N, v, states[t][l], dt * (t + 1)) for t in range(nT)])
         m = np.abs(ref) > 1e-12
         assert (np.abs(rows - ref)[m] / np.abs(ref)[m]).max() < 1e-6, a
+
+
+def test_abi_table_cache_holds_a_three_species_step(spb, orc, capfd):
+    """A three-species mixture walks its 9 pair tables every step
+    (exec/boltz.c:193-198): the reference interface keeps all of them on the
+    GPU (no eviction warning, so no re-upload) and every ComputeQ matches the
+    oracle with its own table, in two rounds."""
+    from conftest import rel_l2
+    N, Lv = 8, 5.0
+    v, eta = orc.grid_0d(N, Lv)
+    spb.initialize_coll(N, Lv, v, eta)
+    tables = [orc.weights_iso(N, eta, Lv, 1.0, m_i=1.0 + i % 3, m_j=1.0 + i // 3) for i in range(9)]
+    rows = [spb.WeightRows(W) for W in tables]   # one conv_weights pointer per pair, kept alive
+    f = orc.bkw_initial(N, v).ravel()
+    g = orc.maxwellian(v, 0.9, np.array([0.2, 0.0, 0.0]), 1.1).ravel()
+    capfd.readouterr()
+    for _ in range(2):
+        for W, R in zip(tables, rows):
+            assert rel_l2(spb.ComputeQ(f, g, R), orc.compute_q(N, v, eta, W, f, g)) < 1e-12
+    assert "evicting" not in capfd.readouterr().err
+    spb.dealloc_coll()
