@@ -604,6 +604,17 @@ def surrogate_run(spb, form="fp64", ncell=4096, train_cells=2048, steps=2500, ba
 
 
 def _cpu_model():
+    """lscpu's model name and socket count (/proc/cpuinfo's model name without lscpu)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        info = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        model = info.get("Model name", "").strip()
+        sockets = info.get("Socket(s)", "").strip()
+        if model:
+            return f"{model}, {sockets} socket(s)" if sockets else model
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
