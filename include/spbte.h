@@ -38,8 +38,8 @@ enum {
  * streams W as float (half the HBM bytes) and widens it to FP64 in registers;
  * FP16X2 streams W as two FP16 halves (hi + lo) with a power-of-two scale per
  * row, widened to FP64; FP32A streams W as float and also computes the products
- * in FP32: the batched (streaming) kernel sums 4 steps in FP32 and adds each
- * partial sum into an FP64 accumulator, the tiled kernel uses compensated
+ * in FP32: the batched (streaming) kernel sums 8 to 16 steps in FP32 and adds
+ * each partial sum into an FP64 accumulator, the tiled kernel uses compensated
  * (TwoSum) FP32 sums.  Transforms, moments and the projection are FP64 in
  * every form. */
 enum { SPBTE_FORM_FP64 = 0, SPBTE_FORM_FP32 = 1, SPBTE_FORM_FP16X2 = 2, SPBTE_FORM_FP32A = 3 };

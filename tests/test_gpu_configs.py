@@ -181,9 +181,9 @@ def test_shock_cells_n32_against_oracle(spb, orc, shock32):
 # large gain and loss terms and the rounding of W, 2^-24 in FP32, is amplified
 # by their ratio): the bound each form meets on the whole batch (norm of the
 # difference over the norm of Q).  Measured on an MI355X: fp32 2.72e-5, fp16x2
-# 4.28e-5, fp32a 8.60e-5 on the streaming kernel and 1.09e-4 on the tiled
-# kernel (compensated sums); none meets 1e-6 here.
-SHOCK_FORM_BOUNDS = {"fp32": 4e-5, "fp16x2": 6e-5, "fp32a": 1.2e-4, "fp32a-tiled": 1.5e-4}
+# 4.28e-5, fp32a 1.07e-4 on the streaming kernel (16-term FP32 partial sums)
+# and 1.09e-4 on the tiled kernel (compensated sums); none meets 1e-6 here.
+SHOCK_FORM_BOUNDS = {"fp32": 4e-5, "fp16x2": 6e-5, "fp32a": 1.6e-4, "fp32a-tiled": 1.6e-4}
 
 
 @pytest.mark.parametrize("form", ["fp32", "fp16x2", "fp32a", "fp32a-tiled"])
