@@ -10,7 +10,7 @@ projection (src/conserve.c:207-264), for all 512 problems; the inputs are
 resident in HBM before timing starts.
 
 `value` = Q evaluations per second with the weight table streamed as FP32 and
-the complex products in FP32 (FP32 partial sums of 16 terms folded into FP64
+the complex products in FP32 (FP32 partial sums of 32 terms folded into FP64
 accumulators).  Alongside, each with its precision form and its error against
 FP64 stated: the same batch in FP64 and in the plain FP64 form (one cell per
 pass over the table, the reference method); the BKW relaxation at N=16; the

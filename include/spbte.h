@@ -38,7 +38,7 @@ enum {
  * streams W as float (half the HBM bytes) and widens it to FP64 in registers;
  * FP16X2 streams W as two FP16 halves (hi + lo) with a power-of-two scale per
  * row, widened to FP64; FP32A streams W as float and also computes the products
- * in FP32: the batched (streaming) kernel sums 8 to 16 steps in FP32 and adds
+ * in FP32: the batched (streaming) kernel sums 8 to 32 steps in FP32 and adds
  * each partial sum into an FP64 accumulator, the tiled kernel uses compensated
  * (TwoSum) FP32 sums.  Transforms, moments and the projection are FP64 in
  * every form. */

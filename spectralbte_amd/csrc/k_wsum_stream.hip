@@ -23,10 +23,10 @@
 //    wave-wide load reads 1 KiB of contiguous HBM, in consumption order.
 // Precision forms: 0 FP64 W, 1 FP32 W, 2 FP16x2 W with FP64 arithmetic; 3 FP32 W
 // with FP32 F, G (8-byte LDS entries, half the LDS), FP32 products and FP32
-// partial sums added into FP64 accumulators every 16 steps in 8-step loop
+// partial sums added into FP64 accumulators every 32 steps in 8-step loop
 // bodies (N=32, N=16) and every 8 steps in longer ones (N=24).  Folding every
-// 4 steps cost 4.7% more time on the 512-problem N=32 batch for a largest
-// error against FP64 of 2.5e-7 instead of 2.8e-7.
+// 4 steps cost 6% more time on the 512-problem N=32 batch for a largest
+// error against FP64 of 2.5e-7 instead of 2.9e-7.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "spbte_internal.h"
@@ -280,7 +280,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
           }
         }
         if constexpr (FORM == 3) {
-          if ((jb + 4) % 8 != 0 || !fold_body) return;  // 8 (16 in 8-step bodies) steps per FP32 partial sum
+          if ((jb + 4) % 8 != 0 || !fold_body) return;  // 8 (32 in 8-step bodies) steps per FP32 partial sum
           // fold the FP32 partial sums into FP64
 #pragma unroll
           for (int b = 0; b < TB; ++b)
@@ -294,15 +294,18 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       };
       // The body is a whole number of window periods (TR) and of chunk pairs,
       // so window slots and the wA/wB roles are the same at every iteration.
-      int base = k0 - 1 + N;  // (k0 - 1 - cz_body) mod N, cz_body = it * BS
+      // (k0 - 1 - cz_body) mod N, cz_body = it * BS; never negative, so the
+      // division by the line-storage group below is a plain shift
+      unsigned base = (unsigned)(k0 - 1 + N);
 #pragma unroll 1
       for (int it = 0; it < N / BS; ++it) {
-        base = base >= N ? base - N : base;
+        base = base >= (unsigned)N ? base - N : base;
         const C *fl[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-          for (int b = 0; b < TB; ++b) fl[p][b] = Gm::TL ? line[p][b] + (BS + base) / Gm::G : line[p][b] + BS + base;
+          for (int b = 0; b < TB; ++b)
+            fl[p][b] = Gm::TL ? line[p][b] + ((unsigned)BS + base) / (unsigned)Gm::G : line[p][b] + BS + base;
         // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar
         // loads), or the staged x-plane in LDS
         const C *gz[NP][TB];
@@ -312,9 +315,9 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
           for (int b = 0; b < TB; ++b)
             gz[p][b] = SG ? Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS
                           : sG + (size_t)(p * TB + b) * N2 + (size_t)cy * N + it * BS;
-        // 8-step bodies fold every other body (16-term partial sums), longer
+        // 8-step bodies fold every fourth body (32-term partial sums), longer
         // bodies every 8 steps; the last body of a line always folds
-        fold_body = (BS >= 16) || (it & 1) || (it == N / BS - 1);
+        fold_body = (BS >= 16) || ((it & 3) == 3) || (it == N / BS - 1);
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
           const int c4 = c4base + it * (BS / 4) + k;

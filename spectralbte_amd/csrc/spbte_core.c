@@ -711,7 +711,8 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * one x-plane per slice: a single cell's FP64 evaluation then streams its
  * table in 48 us instead of 99). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
-  static const spbte_tile s32[] = {{4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}}, s32m64[] = {{8, 2, 1, 1}};
+  static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}},
+                         s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s24ma[] = {{3, 3, 2, 1}, {3, 3, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
@@ -719,7 +720,7 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   const spbte_tile *c = NULL;
   int nc = 0;
   if (N == 32) {
-    if (nprod == 1) { c = s32; nc = 2; }
+    if (nprod == 1) { c = s32; nc = 3; }
     else if (form == SPBTE_FORM_FP64) { c = s32m64; nc = 1; }
     else { c = s32m; nc = 1; }
   } else if (N == 24) {
