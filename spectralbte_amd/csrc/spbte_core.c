@@ -703,7 +703,8 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
 
 /* Streaming kernel defaults (measured on 256-512 cells, MI355X): FP64 weights
  * favour 8 rows per lane for the two-product sum, FP32 weights 4 rows and 4
- * waves; the FP32-product maxPreserve sum at N=24 takes 3 rows x 3 waves with
+ * waves; the one-product (ComputeQ) sum at N=32 four cells per workgroup when
+ * the batch has them; the FP32-product maxPreserve sum at N=24 takes 3 rows x 3 waves with
  * two problems per lane (31.7 ms for 512 problems against 44.2 ms on the tiled
  * kernel).  Unsupported candidates are skipped.
  * Batches too small to give 2048 workgroups split the summed x-planes into up
