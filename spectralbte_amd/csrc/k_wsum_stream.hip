@@ -128,7 +128,12 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   if ((blockIdx.x >> 3) >= per || vid >= total) return;  // the few padding workgroups of the last slice
   const int rt = vid / (csplit * ncg), rem = vid - rt * (csplit * ncg);
   const int cs = rem / ncg, cg = rem - cs * ncg;
-  const int cx0 = cs * (N / csplit), cx1 = cx0 + N / csplit;
+  // slices: up to N take whole x-planes (N / csplit each); beyond N every
+  // x-plane's summed y-lines are split as well (csplit / N pieces of it)
+  const int ny = csplit > N ? csplit / N : 1;
+  const int cx0 = csplit > N ? cs / ny : cs * (N / csplit);
+  const int cx1 = csplit > N ? cx0 + 1 : cx0 + N / csplit;
+  const int cy0 = csplit > N ? (cs - cx0 * ny) * (N / ny) : 0, cy1 = cy0 + N / ny;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t wg_row0 = (size_t)rt * RWG;
   const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
@@ -173,7 +178,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   // Two static register sets alternate over the 4-column chunks of W (a body
   // holds an even number of chunks), each loaded four cz steps before use.
   u32x4 wA[TR][UPW], wB[TR][UPW];
-  wload(cx0 * N * (N / 4), wA);
+  wload((cx0 * N + cy0) * (N / 4), wA);
 
   for (int cx = cx0; cx < cx1; ++cx) {
     __syncthreads();
@@ -196,7 +201,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       }
     __syncthreads();
 
-    for (int cy = 0; cy < N; ++cy) {
+    for (int cy = cy0; cy < cy1; ++cy) {
       int sy = ry + N / 2 - cy;
       sy = sy < 0 ? sy + N : (sy > N - 1 ? sy - N : sy);
       const C *line[NP][TB];
@@ -459,17 +464,24 @@ __global__ void __launch_bounds__(256) k_permute_w(const double *__restrict__ W,
 
 }  // namespace
 
+extern "C" int spbte_csplit_valid(int N, int csplit) {
+  if (csplit < 1) return 0;
+  if (csplit <= N) return N % csplit == 0;
+  return csplit % N == 0 && N % (csplit / N) == 0;
+}
+
 extern "C" int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb) {
   return sfind(N, form, nprod, tr, waves, tb) != nullptr;
 }
 
 // Launch with the permuted table Wp built for the same tr (spbte_launch_permute_weights).
-// csplit: slices of the summed x-planes (divides N); csplit > 1 needs Qp with
-// room for csplit * ncell * N^3 double2.
+// csplit: slices of the summed modes -- a divisor of N (whole x-planes) or N
+// times a divisor of N (x-planes cut into y-line ranges); csplit > 1 needs Qp
+// with room for csplit * ncell * N^3 double2.
 extern "C" int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
                                         const float *rowscale, int csplit, void *Qp, hipStream_t s) {
   const SEntry *e = sfind(N, form, a->nprod, tr, waves, tb);
-  if (!e || csplit < 1 || N % csplit || (csplit > 1 && !Qp)) return (int)hipErrorInvalidValue;
+  if (!e || !spbte_csplit_valid(N, csplit) || (csplit > 1 && !Qp)) return (int)hipErrorInvalidValue;
   return e->fn(a, Wp, rowscale, csplit, Qp, s);
 }
 

@@ -708,9 +708,11 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * two problems per lane (31.7 ms for 512 problems against 44.2 ms on the tiled
  * kernel).  Unsupported candidates are skipped.
  * Batches too small to give 2048 workgroups split the summed x-planes into up
- * to 8 slices (16 at N=16, where one to four cells take one row per lane and
- * one x-plane per slice: a single cell's FP64 evaluation then streams its
- * table in 48 us instead of 99). */
+ * to 8 slices; at N=16, where one to four cells take one row per lane, into
+ * as many as give 2048 workgroups, cutting x-planes into y-line ranges if
+ * need be (one cell: 32 slices of half an x-plane, its FP64 table streamed in
+ * 41.6 us against 48.3 with 16 whole planes and 99 with 8 slices of 4 rows;
+ * 64 slices took 56.8 us, the F staging outweighing the extra waves). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}},
                          s32m64[] = {{8, 2, 1, 1}};
@@ -733,14 +735,13 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
     else { c = nprod == 2 ? s16m : s16; nc = 2; }
   }
   else return 0;
-  const int kmax = N <= 16 ? N : 8;
   for (int i = 0; i < nc; i++) {
     if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
     if (c[i].tb > ncell && i + 1 < nc) continue;
     *t = c[i];
     const long wgs = stream_wgs(N, ncell, t);
     int k = 1;
-    while (k < kmax && wgs * k < 2048 && N % (2 * k) == 0) k *= 2;
+    while (wgs * k < 2048 && spbte_csplit_valid(N, 2 * k) && (N <= 16 || 2 * k <= 8)) k *= 2;
     t->kind = k;
     return 1;
   }
@@ -757,7 +758,7 @@ static int pick_tile(spbte_op *op, int ncell, int form, int nprod, spbte_tile *t
 
 static int tile_supported(int N, int form, int nprod, const spbte_tile *t) {
   if (t->kind >= 1)
-    return t->kind <= N && N % t->kind == 0 && spbte_stream_tile_supported(N, form, nprod, t->tr, t->lines, t->tb);
+    return spbte_csplit_valid(N, t->kind) && spbte_stream_tile_supported(N, form, nprod, t->tr, t->lines, t->tb);
   return spbte_wsum_tile_supported(N, form, nprod, t);
 }
 
@@ -1088,7 +1089,9 @@ static const spbte_tile k_cand[] = {
     {4, 4, 1, 2}, {4, 4, 1, 4}, {4, 4, 1, 8}, {4, 4, 2, 2}, {4, 4, 2, 4}, {4, 4, 2, 8}, {4, 2, 1, 4}, {4, 2, 1, 8},
     {4, 2, 2, 4}, {8, 2, 1, 4}, {8, 2, 1, 8}, {4, 2, 1, 2}, {4, 9, 1, 2}, {4, 9, 1, 4},
     /* one or a few cells at N=16: narrower rows, up to one x-plane per slice */
-    {4, 1, 1, 8}, {4, 1, 1, 16}, {2, 1, 1, 8}, {2, 1, 1, 16}, {1, 1, 1, 8}, {1, 1, 1, 16}, {8, 2, 1, 16}};
+    {4, 1, 1, 8}, {4, 1, 1, 16}, {2, 1, 1, 8}, {2, 1, 1, 16}, {1, 1, 1, 8}, {1, 1, 1, 16}, {8, 2, 1, 16},
+    /* and x-planes cut into y-line ranges */
+    {1, 1, 1, 32}, {1, 1, 1, 64}, {2, 1, 1, 32}, {2, 1, 1, 64}, {4, 1, 1, 64}, {4, 1, 2, 64}};
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");

@@ -81,6 +81,9 @@ int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 int spbte_launch_fill_pattern(void *p, size_t n, int single, hipStream_t s);
 int spbte_wsum_tile_supported(int N, int form, int nprod, const spbte_tile *tile);
 int spbte_stream_tile_supported(int N, int form, int nprod, int tr, int waves, int tb);
+/* 1 if the streaming kernel can split the summed modes into csplit slices:
+ * a divisor of N (x-planes) or N times a divisor of N (y-line ranges). */
+int spbte_csplit_valid(int N, int csplit);
 int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const spbte_wsum_args *a, const void *Wp,
                              const float *rowscale, int csplit, void *Qp, hipStream_t s);
 int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,

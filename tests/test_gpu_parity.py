@@ -97,7 +97,8 @@ def test_qhat_tiles_agree(spb, orc, N):
     op = spb.Operator(N, 5.0, v, eta)
     op.set_weights(W)
     ref = np.stack([orc.qhat(N, W, F[b], G[b]).ravel() for b in range(ncell)])
-    tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 8, 2, 0), (4, 8, 4, 0), (4, 1, 1, 1), (4, 1, 2, 1)],
+    tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 8, 2, 0), (4, 8, 4, 0), (4, 1, 1, 1), (4, 1, 2, 1),
+                  (1, 1, 1, 16), (1, 1, 1, 64), (2, 1, 1, 32), (4, 1, 2, 64)],   # x-planes cut into y-line ranges
              24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 8, 4, 0), (6, 3, 1, 1), (3, 3, 1, 1), (3, 3, 2, 1), (4, 2, 1, 1),
                   (4, 2, 2, 1), (8, 1, 1, 1)]}[N]
     for t in tiles:
