@@ -51,6 +51,36 @@ __device__ __forceinline__ double maxwellian_at(const double *mom, const double 
   return pre * exp(-(0.5 / T) * (dx * dx + dy * dy + dz * dz));
 }
 
+// rho, u, T and the Maxwellian prefactor of one cell from the sums over its
+// grid of w f, w v_x f, w v_y f, w v_z f, w |v|^2 f
+// (src/momentRoutines.c:58-72, :116-142, :168-183).  The temperature's
+// sum of w |v - u|^2 f is taken as sum w |v|^2 f - |sum w v f|^2 / sum w f,
+// which equals the reference's two-pass sum up to rounding.
+__device__ void moments_finish(const double (&s)[5], double hv, double mass, double kb, double *m) {
+  const double dv3 = hv * hv * hv;
+  const double rho = mass * dv3 * s[0];
+  const double ux = s[1] / s[0], uy = s[2] / s[0], uz = s[3] / s[0];
+  const double c2 = s[4] - (s[1] * ux + s[2] * uy + s[3] * uz);
+  const double T = (mass * mass / kb) * dv3 * c2 / (3.0 * rho);
+  m[0] = rho;
+  m[1] = ux;
+  m[2] = uy;
+  m[3] = uz;
+  m[4] = T;
+  m[5] = rho * pow(0.5 / (M_PI * T), 1.5);
+}
+
+// The same from the cell's N (<= 64) per-plane partials, summed across the
+// lanes of the calling wave; m is written by lane 0.
+__device__ void moments_wave(int N, const double *__restrict__ part, double hv, double mass, double kb, double *m) {
+  const int l = threadIdx.x & 63;
+  double s[5];
+  for (int q = 0; q < 5; ++q) s[q] = l < N ? part[(size_t)l * 5 + q] : 0.0;
+  for (int q = 0; q < 5; ++q)
+    for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off, 64);
+  if (l == 0) moments_finish(s, hv, mass, kb, m);
+}
+
 // z then y transform of one x-plane; T2 goes to scratch (FP64 complex).
 __device__ void plane_zy(int N, const double2 *__restrict__ M, double2 *X, double2 *T1, double2 *__restrict__ dst) {
   const int n2 = N * N;
@@ -140,6 +170,25 @@ __global__ void __launch_bounds__(XT) k_xform_b(int N, int ncell, const double2 
       else ((float2 *)dst)[o] = make_float2((float)acc.x, (float)acc.y);
     }
   }
+}
+
+template <int NV>
+__device__ void block_sum(double (&x)[NV], double *red /* NV * (XT/64) */) {
+  for (int q = 0; q < NV; ++q) {
+    double s = x[q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    x[q] = s;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int q = 0; q < NV; ++q) red[q * (XT / 64) + wave] = x[q];
+  __syncthreads();
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < XT / 64; ++w) s += red[q * (XT / 64) + w];
+    x[q] = s;
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------- FFTs
@@ -259,16 +308,21 @@ __device__ void load_tabs(FftTabs &t, const double2 *__restrict__ g) {
 // of (cell, job) times p(i) p(j) p(k), then z and y FFTs, to scratch.
 template <int N>
 __global__ void __launch_bounds__(XT) k_fft_a_real(XformJobs jobs, int ncell, const double2 *__restrict__ tabs,
-                                                   const double *__restrict__ v, double2 *__restrict__ scratch, double dsgn) {
+                                                   const double *__restrict__ v, double2 *__restrict__ scratch, double dsgn,
+                                                   double hv, double mass, double kb) {
   __shared__ double2 A[N * N], B[N * N];
   __shared__ FftTabs t;
+  __shared__ double smom[SPBTE_MOM_STRIDE];
   constexpr int n2 = N * N;
   constexpr size_t n3 = (size_t)n2 * N;
   const int i = blockIdx.x, cell = blockIdx.y, jb = blockIdx.z;
   const spbte_xform_job &job = jobs.j[jb];
   load_tabs<N>(t, tabs);
+  // the Maxwellian's moments: from the cell's per-plane partial sums, or given
+  if (job.part && job.mode != 0 && threadIdx.x < 64)
+    moments_wave(N, job.part + (size_t)cell * N * 5, hv, mass, kb, smom);
   __syncthreads();
-  const double *mom = job.mom ? job.mom + (size_t)cell * SPBTE_MOM_STRIDE : nullptr;
+  const double *mom = job.part ? smom : (job.mom ? job.mom + (size_t)cell * SPBTE_MOM_STRIDE : nullptr);
   const double2 pi_ = t.p[i];
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int j = e / N, k = e - j * N;
@@ -287,11 +341,13 @@ __global__ void __launch_bounds__(XT) k_fft_a_real(XformJobs jobs, int ncell, co
   for (int e = threadIdx.x; e < n2; e += XT) dst[e] = r[e];
 }
 
-// Pass A with FFTs, complex FP64 inputs.
+// Pass A with FFTs, complex FP64 inputs: the sum of nsl sets ncell N^3 apart
+// (added in set order).  in may alias scratch: a workgroup's plane of set 0 is
+// the only part of in that overlaps what it writes, and it reads all of its
+// input before it writes.
 template <int N>
-__global__ void __launch_bounds__(XT) k_fft_a_cplx(const double2 *__restrict__ in, int ncell,
-                                                   const double2 *__restrict__ tabs, double2 *__restrict__ scratch,
-                                                   double dsgn) {
+__global__ void __launch_bounds__(XT) k_fft_a_cplx(const double2 *in, int nsl, int ncell,
+                                                   const double2 *__restrict__ tabs, double2 *scratch, double dsgn) {
   __shared__ double2 A[N * N], B[N * N];
   __shared__ FftTabs t;
   constexpr int n2 = N * N;
@@ -300,9 +356,23 @@ __global__ void __launch_bounds__(XT) k_fft_a_cplx(const double2 *__restrict__ i
   load_tabs<N>(t, tabs);
   __syncthreads();
   const double2 pi_ = t.p[i];
+  const size_t sstride = (size_t)ncell * n3;
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int j = e / N, k = e - j * N;
-    A[e] = cmul(in[cell * n3 + (size_t)i * n2 + e], cmul(cmul(pi_, t.p[j]), t.p[k]));
+    const size_t o = cell * n3 + (size_t)i * n2 + e;
+    double2 x = in[o];
+    for (int sl = 1; sl < nsl; sl += 8) {  // up to eight loads in flight, added in set order
+      double2 y[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) y[u] = sl + u < nsl ? in[(size_t)(sl + u) * sstride + o] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (sl + u < nsl) {
+          x.x += y[u].x;
+          x.y += y[u].y;
+        }
+    }
+    A[e] = cmul(x, cmul(cmul(pi_, t.p[j]), t.p[k]));
   }
   __syncthreads();
   double2 *r = fft_axis<N>(A, B, N, 1, t.tw, dsgn);
@@ -317,7 +387,8 @@ __global__ void __launch_bounds__(XT) k_fft_a_cplx(const double2 *__restrict__ i
 template <int N, int OUT>
 __global__ void __launch_bounds__(XT) k_fft_b(int ncell, const double2 *__restrict__ tabs, double pre,
                                               const double2 *__restrict__ scratch, XformJobs jobs, void *out0,
-                                              double dsgn) {
+                                              double dsgn, const double *__restrict__ v, const double *__restrict__ w,
+                                              double pm, double *__restrict__ ppart) {
   __shared__ double2 A[N * N], B[N * N];
   __shared__ FftTabs t;
   constexpr int n2 = N * N;
@@ -333,6 +404,9 @@ __global__ void __launch_bounds__(XT) k_fft_b(int ncell, const double2 *__restri
   double2 *r = fft_axis<N>(A, B, 1, N, t.tw, dsgn);        // x: sequences c, elements i (stride N)
   void *dst = jobs.j[jb].dst ? jobs.j[jb].dst : out0;
   const double2 qb = t.q[b];
+  // OUT == 2 with ppart: the one-species projection's partial sums of this
+  // y-plane, p {1, v_x, v_y, v_z, |v|^2 / 2} q with p = w h_v^3 m
+  double ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int a = e / N, c = e - a * N;
     const size_t o = cell * n3 + (size_t)a * n2 + b * N + c;
@@ -341,29 +415,27 @@ __global__ void __launch_bounds__(XT) k_fft_b(int ncell, const double2 *__restri
     f.y *= pre;
     if (OUT == 0) ((double2 *)dst)[o] = f;
     else if (OUT == 1) ((float2 *)dst)[o] = make_float2((float)f.x, (float)f.y);
-    else ((double *)dst)[o] = f.x;
+    else {
+      ((double *)dst)[o] = f.x;
+      if (ppart) {
+        const double p = w[a] * w[b] * w[c] * pm * f.x;
+        ps[0] += p;
+        ps[1] += p * v[a];
+        ps[2] += p * v[b];
+        ps[3] += p * v[c];
+        ps[4] += p * 0.5 * (v[a] * v[a] + v[b] * v[b] + v[c] * v[c]);
+      }
+    }
+  }
+  if (OUT == 2 && ppart) {
+    __shared__ double red[5 * (XT / 64)];
+    block_sum<5>(ps, red);
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 5; ++q) ppart[((size_t)cell * N + b) * 5 + q] = ps[q];
   }
 }
 
 // ---------------------------------------------------------------- reductions
-template <int NV>
-__device__ void block_sum(double (&x)[NV], double *red /* NV * (XT/64) */) {
-  for (int q = 0; q < NV; ++q) {
-    double s = x[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    x[q] = s;
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0)
-    for (int q = 0; q < NV; ++q) red[q * (XT / 64) + wave] = x[q];
-  __syncthreads();
-  for (int q = 0; q < NV; ++q) {
-    double s = 0.0;
-    for (int w = 0; w < XT / 64; ++w) s += red[q * (XT / 64) + w];
-    x[q] = s;
-  }
-  __syncthreads();
-}
 
 // Moments for the Maxwellian split (src/collisions.c:95-98 with
 // src/momentRoutines.c:58-72, :116-142, :168-183): rho, u, T, prefactor.
@@ -504,10 +576,7 @@ __global__ void __launch_bounds__(XT) k_moments_part(int N, const double *__rest
     for (int q = 0; q < PS; ++q) part[((size_t)cell * N + i) * PS + q] = s[q];
 }
 
-// rho, u, T and the Maxwellian prefactor from a cell's N partials
-// (src/momentRoutines.c:58-72, :116-142, :168-183).  The temperature's
-// sum of w |v - u|^2 f is taken as sum w |v|^2 f - |sum w v f|^2 / sum w f,
-// which equals the reference's two-pass sum up to rounding.
+// rho, u, T and the Maxwellian prefactor from a cell's N partials.
 __global__ void __launch_bounds__(64) k_moments_fin(int N, int ncell, const double *__restrict__ part, double hv,
                                                     double mass, double kb, double *__restrict__ mom) {
   const int cell = blockIdx.x * 64 + threadIdx.x;
@@ -515,18 +584,7 @@ __global__ void __launch_bounds__(64) k_moments_fin(int N, int ncell, const doub
   double s[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = 0; i < N; ++i)
     for (int q = 0; q < PS; ++q) s[q] += part[((size_t)cell * N + i) * PS + q];
-  const double dv3 = hv * hv * hv;
-  const double rho = mass * dv3 * s[0];
-  const double ux = s[1] / s[0], uy = s[2] / s[0], uz = s[3] / s[0];
-  const double c2 = s[4] - (s[1] * ux + s[2] * uy + s[3] * uz);
-  const double T = (mass * mass / kb) * dv3 * c2 / (3.0 * rho);
-  double *m = mom + (size_t)cell * SPBTE_MOM_STRIDE;
-  m[0] = rho;
-  m[1] = ux;
-  m[2] = uy;
-  m[3] = uz;
-  m[4] = T;
-  m[5] = rho * pow(0.5 / (M_PI * T), 1.5);
+  moments_finish(s, hv, mass, kb, mom + (size_t)cell * SPBTE_MOM_STRIDE);
 }
 
 // One species: partial sums of p {1, v_x, v_y, v_z, |v|^2/2} Q over x-plane i,
@@ -562,22 +620,26 @@ __global__ void __launch_bounds__(XT) k_proj_apply(int N, const double *__restri
   __shared__ double lam[PS];
   const int i = blockIdx.x, cell = blockIdx.y;
   const int n2 = N * N;
-  if (threadIdx.x == 0) {
-    double b[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int x = 0; x < N; ++x)
-      for (int q = 0; q < PS; ++q) b[q] += part[((size_t)cell * N + x) * PS + q];
-    for (int k = 0; k < PS - 1; ++k) {
-      const int p = pa.pivot[k];
-      if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
-      for (int r = k + 1; r < PS; ++r) b[r] -= pa.LU[r * PS + k] * b[k];
+  if (threadIdx.x < 64) {  // wave 0 adds the partials (N <= 64), lane 0 solves
+    const int l = threadIdx.x;
+    double b[PS];
+    for (int q = 0; q < PS; ++q) b[q] = l < N ? part[((size_t)cell * N + l) * PS + q] : 0.0;
+    for (int q = 0; q < PS; ++q)
+      for (int off = 32; off > 0; off >>= 1) b[q] += __shfl_xor(b[q], off, 64);
+    if (l == 0) {
+      for (int k = 0; k < PS - 1; ++k) {
+        const int p = pa.pivot[k];
+        if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
+        for (int r = k + 1; r < PS; ++r) b[r] -= pa.LU[r * PS + k] * b[k];
+      }
+      b[PS - 1] = b[PS - 1] / pa.LU[(PS - 1) * PS + PS - 1];
+      for (int r = PS - 2; r >= 0; --r) {
+        double sum = 0.0;
+        for (int c = r + 1; c < PS; ++c) sum += pa.LU[r * PS + c] * b[c];
+        b[r] = 1.0 / pa.LU[r * PS + r] * (b[r] - sum);
+      }
+      for (int q = 0; q < PS; ++q) lam[q] = b[q];
     }
-    b[PS - 1] = b[PS - 1] / pa.LU[(PS - 1) * PS + PS - 1];
-    for (int r = PS - 2; r >= 0; --r) {
-      double sum = 0.0;
-      for (int c = r + 1; c < PS; ++c) sum += pa.LU[r * PS + c] * b[c];
-      b[r] = 1.0 / pa.LU[r * PS + r] * (b[r] - sum);
-    }
-    for (int q = 0; q < PS; ++q) lam[q] = b[q];
   }
   __syncthreads();
   double *qp = Q + (size_t)cell * n2 * N + (size_t)i * n2;
@@ -598,30 +660,32 @@ void launch_fft_fwd(const spbte_grid_dev *g, const XformJobs &J, int njobs, int 
                     hipStream_t s) {
   const double2 *tf = (const double2 *)g->d_fft;  // forward p, q, tw
   hipLaunchKernelGGL(k_fft_a_real<N>, dim3(N, ncell, njobs), dim3(XT), 0, s, J, ncell, tf, (const double *)g->d_v,
-                     (double2 *)scratch, -1.0);
+                     (double2 *)scratch, -1.0, g->hv, g->mass, g->kb);
   if (out_fp32)
     hipLaunchKernelGGL((k_fft_b<N, 1>), dim3(N, ncell, njobs), dim3(XT), 0, s, ncell, tf, g->pre_fwd,
-                       (const double2 *)scratch, J, nullptr, -1.0);
+                       (const double2 *)scratch, J, nullptr, -1.0, nullptr, nullptr, 0.0, nullptr);
   else
     hipLaunchKernelGGL((k_fft_b<N, 0>), dim3(N, ncell, njobs), dim3(XT), 0, s, ncell, tf, g->pre_fwd,
-                       (const double2 *)scratch, J, nullptr, -1.0);
+                       (const double2 *)scratch, J, nullptr, -1.0, nullptr, nullptr, 0.0, nullptr);
 }
 
-// Complex cube transform (invert 0/1, OUT 0 = complex, 2 = real part).
+// Complex cube transform (invert 0/1, OUT 0 = complex, 2 = real part, with
+// the projection's partials into ppart when it is not null).
 template <int N>
-void launch_fft_cplx(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert, int real_out,
-                     void *scratch, hipStream_t s) {
+void launch_fft_cplx(const spbte_grid_dev *g, const void *in, int nsl, void *out, int ncell, int invert, int real_out,
+                     void *scratch, double *ppart, hipStream_t s) {
   const double2 *t = (const double2 *)g->d_fft + (invert ? 3 * N : 0);
   const double dsgn = invert ? 1.0 : -1.0, pre = invert ? g->pre_inv : g->pre_fwd;
   XformJobs J = {};
-  hipLaunchKernelGGL(k_fft_a_cplx<N>, dim3(N, ncell, 1), dim3(XT), 0, s, (const double2 *)in, ncell, t,
+  hipLaunchKernelGGL(k_fft_a_cplx<N>, dim3(N, ncell, 1), dim3(XT), 0, s, (const double2 *)in, nsl, ncell, t,
                      (double2 *)scratch, dsgn);
+  const double pm = g->hv * g->hv * g->hv * g->mass;
   if (real_out)
     hipLaunchKernelGGL((k_fft_b<N, 2>), dim3(N, ncell, 1), dim3(XT), 0, s, ncell, t, pre, (const double2 *)scratch, J,
-                       out, dsgn);
+                       out, dsgn, (const double *)g->d_v, (const double *)g->d_w, pm, ppart);
   else
     hipLaunchKernelGGL((k_fft_b<N, 0>), dim3(N, ncell, 1), dim3(XT), 0, s, ncell, t, pre, (const double2 *)scratch, J,
-                       out, dsgn);
+                       out, dsgn, nullptr, nullptr, 0.0, nullptr);
 }
 
 // Runs fn<N>() for the grid's N if it has FFT kernels; false otherwise.
@@ -654,6 +718,13 @@ extern "C" int spbte_launch_moments(const spbte_grid_dev *g, const double *f, do
   return (int)hipGetLastError();
 }
 
+extern "C" int spbte_launch_moment_partials(const spbte_grid_dev *g, const double *f, int ncell, double *part,
+                                            hipStream_t s) {
+  hipLaunchKernelGGL(k_moments_part, dim3(g->N, ncell), dim3(XT), 0, s, g->N, f, (const double *)g->d_v,
+                     (const double *)g->d_w, part);
+  return (int)hipGetLastError();
+}
+
 extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform_job *jobs, int njobs, int ncell,
                                       int out_fp32, void *scratch, hipStream_t s) {
   if (njobs < 1 || njobs > 4) return (int)hipErrorInvalidValue;
@@ -663,6 +734,8 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
         launch_fft_fwd<decltype(n)::value>(g, J, njobs, ncell, out_fp32, scratch, s);
       }))
     return (int)hipGetLastError();
+  for (int q = 0; q < njobs; ++q)
+    if (jobs[q].part) return (int)hipErrorInvalidValue;  // the matrix form takes moments, not partials
   const int N = g->N;
   const size_t lds_a = 2 * (size_t)N * N * sizeof(double2), lds_b = (size_t)N * N * sizeof(double2);
   hipLaunchKernelGGL(k_xform_a_real, dim3(N, ncell, njobs), dim3(XT), lds_a, s, N, J, ncell,
@@ -679,7 +752,7 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
 extern "C" int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert,
                                               void *scratch, hipStream_t s) {
   if (g->d_fft && with_fft_n(g->N, [&](auto n) {
-        launch_fft_cplx<decltype(n)::value>(g, in, out, ncell, invert, 0, scratch, s);
+        launch_fft_cplx<decltype(n)::value>(g, in, 1, out, ncell, invert, 0, scratch, nullptr, s);
       }))
     return (int)hipGetLastError();
   const int N = g->N;
@@ -693,12 +766,14 @@ extern "C" int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const voi
   return (int)hipGetLastError();
 }
 
-extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, double *q, int ncell, void *scratch,
-                                           hipStream_t s) {
+extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell,
+                                           void *scratch, double *proj_part, hipStream_t s) {
+  if (nsl < 1) return (int)hipErrorInvalidValue;
   if (g->d_fft && with_fft_n(g->N, [&](auto n) {
-        launch_fft_cplx<decltype(n)::value>(g, Qh, q, ncell, 1, 1, scratch, s);
+        launch_fft_cplx<decltype(n)::value>(g, Qh, nsl, q, ncell, 1, 1, scratch, proj_part, s);
       }))
     return (int)hipGetLastError();
+  if (proj_part || nsl != 1) return (int)hipErrorInvalidValue;  // the matrix form takes one set, writes no partials
   const int N = g->N;
   const size_t lds_a = 2 * (size_t)N * N * sizeof(double2), lds_b = (size_t)N * N * sizeof(double2);
   XformJobs J = {};
@@ -722,6 +797,18 @@ extern "C" int spbte_launch_project(const spbte_grid_dev *g, int Ns, const doubl
                      pm, (const double *)Q, part);
   hipLaunchKernelGGL(k_proj_apply, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
                      (const double *)g->d_w, pm, pa, Q, (const double *)part);
+  return (int)hipGetLastError();
+}
+
+extern "C" int spbte_launch_project_apply(const spbte_grid_dev *g, const double *masses, const double *LU,
+                                          const int *pivot, double *Q, int ncell, const double *part, hipStream_t s) {
+  ProjArgs pa = {};
+  for (int q = 0; q < PS * PS; ++q) pa.LU[q] = LU[q];
+  for (int q = 0; q < PS; ++q) pa.pivot[q] = pivot[q];
+  pa.masses[0] = masses[0];
+  const double pm = g->hv * g->hv * g->hv * masses[0];
+  hipLaunchKernelGGL(k_proj_apply, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
+                     (const double *)g->d_w, pm, pa, Q, part);
   return (int)hipGetLastError();
 }
 

@@ -382,7 +382,7 @@ int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, in
                        (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
                        (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
                        (double2 *)Qp);
-  if (csplit > 1) {
+  if (csplit > 1 && !a->keep_slices) {
     const size_t n = (size_t)a->ncell * N * N * N;
     hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384)), dim3(256), 0,
                        s, n, csplit, (const double2 *)Qp, (double2 *)a->Qh);

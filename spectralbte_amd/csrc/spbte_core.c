@@ -81,7 +81,7 @@ struct spbte_op {
   void *d_xs;        /* transform scratch: MAXJOBS * cap * n3 double2 */
   void *d_FG;        /* MAXJOBS * cap * n3 double2 (FP32 forms use half) */
   void *d_Qh;        /* cap * n3 double2 */
-  double *d_part;    /* 5 * N * cap: per-plane partial sums of the moments and the projection */
+  double *d_part;    /* 10 * N * cap: per-plane partial sums of the moments (f's, g's) and the projection */
   /* tiles pinned per (form, nprod-1); tr == 0 -> automatic */
   spbte_tile pin[NFORMS][2];
   /* autotuned tiles per (form, nprod-1, ncell bucket) */
@@ -415,7 +415,7 @@ static int ensure_scratch(spbte_op *op, int ncell) {
   HIPCHK(hipMalloc(&op->d_xs, 16 * n * MAXJOBS));
   HIPCHK(hipMalloc(&op->d_FG, 16 * n * MAXJOBS));
   HIPCHK(hipMalloc(&op->d_Qh, 16 * n));
-  HIPCHK(hipMalloc((void **)&op->d_part, sizeof(double) * 5 * (size_t)op->N * ncell));
+  HIPCHK(hipMalloc((void **)&op->d_part, sizeof(double) * 10 * (size_t)op->N * ncell));
   op->cap = ncell;
   op->gen++;
   return SPBTE_OK;
@@ -427,7 +427,7 @@ static int ensure_scratch(spbte_op *op, int ncell) {
  * must be resident and Qp must already hold qp_bytes. */
 typedef struct {
   double *mom; /* 2 * ncell * SPBTE_MOM_STRIDE */
-  double *part; /* 5 * N * ncell, per-plane partial sums */
+  double *part; /* 10 * N * ncell, per-plane partial sums (two sets) */
   void *xs;    /* transform scratch */
   void *FG;    /* transformed operands */
   void *Qh;    /* Qhat, ncell * N^3 double2 */
@@ -797,8 +797,13 @@ int spbte_get_tile(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
 }
 
 /* ---------------------------------------------------------- the operator */
-static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a, const scratch_set *ss) {
+/* The weighted sum's result is left at *qsrc as *nsl sets to add: Qh (one set),
+ * or, with a->keep_slices, a split tile's unsummed slices in Qp. */
+static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_wsum_args *a, const scratch_set *ss,
+                       const void **qsrc, int *nsl) {
   const float *rs;
+  *qsrc = a->Qh;
+  *nsl = 1;
   if (t && t->kind >= 1) {
     const void *Wp = NULL;
     int rc = weights_permuted(op, form, t->tr, !ss->fixed, &Wp, &rs);
@@ -816,6 +821,10 @@ static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_
       }
     }
     LCHK(spbte_launch_wsum_stream(op->N, form, t->tr, t->lines, t->tb, a, Wp, rs, t->kind, Qp, op->s));
+    if (a->keep_slices && t->kind > 1) {
+      *qsrc = Qp;
+      *nsl = t->kind;
+    }
     return SPBTE_OK;
   }
   const void *W;
@@ -827,15 +836,19 @@ static int launch_tile(spbte_op *op, int form, const spbte_tile *t, const spbte_
   return SPBTE_OK;
 }
 
+/* keep_slices: a split tile may leave its slices unsummed (see launch_tile);
+ * qsrc/nsl (NULL unless keep_slices) say where the result is. */
 static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const void *G0, const void *F1, const void *G1,
-                    int ncell, const scratch_set *ss) {
+                    int ncell, const scratch_set *ss, int keep_slices, const void **qsrc, int *nsl) {
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   spbte_tile t;
   int have = pick_tile(op, ncell, form, nprod, &t);
   if (!have && form != SPBTE_FORM_FP64)
     return fail(SPBTE_ERR_UNSUPPORTED, "no reduced-precision weighted-sum kernel for N=%d", op->N);
-  spbte_wsum_args a = {NULL, F0, G0, F1, G1, ss->Qh, ncell, nprod};
-  return launch_tile(op, form, have ? &t : NULL, &a, ss);
+  spbte_wsum_args a = {NULL, F0, G0, F1, G1, ss->Qh, ncell, nprod, keep_slices};
+  const void *q;
+  int n;
+  return launch_tile(op, form, have ? &t : NULL, &a, ss, qsrc ? qsrc : &q, nsl ? nsl : &n);
 }
 
 /* One chunk of spbte_compute_q.  ev_a/ev_b (optional) bracket the weighted-sum
@@ -853,40 +866,55 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
   int nj = 0;
   const void *F0, *G0, *F1 = NULL, *G1 = NULL;
   double *mom_f = ss->mom, *mom_g = ss->mom + (size_t)ncell * SPBTE_MOM_STRIDE;
+  /* FFT grids: the forward transform computes the Maxwellians' moments itself
+   * from per-plane partial sums (f's, then g's, in ss->part), and the inverse
+   * transform writes the projection's partial sums (over f's, dead by then) */
+  const int fused = op->g.d_fft != NULL && spbte_fft_supported(op->N);
+  const double *part_f = fused ? ss->part : NULL;
+  const double *part_g = fused ? ss->part + (size_t)5 * op->N * ncell : NULL;
   if (!mp) {
-    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL};
     F0 = B[0];
     if (self) G0 = B[0];
-    else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1]}; G0 = B[1]; }
+    else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1], NULL}; G0 = B[1]; }
   } else {
     /* Q = sum W [ T(g - M_f)[c] T(f)[s] + T(M_g)[c] T(f - M_f)[s] ] */
-    LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, ss->part, op->s));
-    if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, ss->part, op->s));
-    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0]};        /* F0 = T(f)        */
-    jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1]};       /* F1 = T(f - M_f)  */
+    if (fused) {
+      LCHK(spbte_launch_moment_partials(&op->g, d_f, ncell, (double *)part_f, op->s));
+      if (!self) LCHK(spbte_launch_moment_partials(&op->g, d_g, ncell, (double *)part_g, op->s));
+      mom_f = mom_g = NULL;
+    } else {
+      LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, ss->part, op->s));
+      if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, ss->part, op->s));
+    }
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL};         /* F0 = T(f)        */
+    jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1], part_f};      /* F1 = T(f - M_f)  */
     F0 = B[0];
     F1 = B[1];
     if (self) {
       G0 = B[1];
-      jobs[nj++] = (spbte_xform_job){d_f, mom_f, 2, B[2]};     /* G1 = T(M_f)      */
+      jobs[nj++] = (spbte_xform_job){d_f, mom_f, 2, B[2], part_f};    /* G1 = T(M_f)      */
       G1 = B[2];
     } else {
-      jobs[nj++] = (spbte_xform_job){d_g, mom_f, 1, B[2]};     /* G0 = T(g - M_f)  */
-      jobs[nj++] = (spbte_xform_job){d_g, mom_g, 2, B[3]};     /* G1 = T(M_g)      */
+      jobs[nj++] = (spbte_xform_job){d_g, mom_f, 1, B[2], part_f};    /* G0 = T(g - M_f)  */
+      jobs[nj++] = (spbte_xform_job){d_g, mom_g, 2, B[3], part_g};    /* G1 = T(M_g)      */
       G0 = B[2];
       G1 = B[3];
     }
   }
   LCHK(spbte_launch_xform_fwd(&op->g, jobs, nj, ncell, fg_bytes(form) == 8, ss->xs, op->s));
   if (ev_a) HIPCHK(hipEventRecord(ev_a, op->s));
-  int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell, ss);
+  /* FFT grids: the inverse transform adds the split sum's slices itself */
+  const void *qsrc;
+  int nsl;
+  int rc = run_wsum(op, form, mp ? 2 : 1, F0, G0, F1, G1, ncell, ss, fused, &qsrc, &nsl);
   if (ev_b) HIPCHK(hipEventRecord(ev_b, op->s));
   if (rc) return rc;
-  LCHK(spbte_launch_xform_inv_real(&op->g, ss->Qh, d_Q, ncell, ss->xs, op->s));
-  if (flags & SPBTE_CONSERVE) {
-    if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
-    LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
-  }
+  const int cons = (flags & SPBTE_CONSERVE) != 0;
+  if (cons && op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
+  LCHK(spbte_launch_xform_inv_real(&op->g, qsrc, nsl, d_Q, ncell, ss->xs, cons && fused ? ss->part : NULL, op->s));
+  if (cons && fused) LCHK(spbte_launch_project_apply(&op->g, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
+  else if (cons) LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
   return SPBTE_OK;
 }
 
@@ -940,7 +968,7 @@ int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, 
   const int nj = mp ? (self ? 3 : 4) : (self ? 1 : 2);
   const size_t n = (size_t)ncell * op->n3;
   const size_t b_mom = al256(sizeof(double) * 2 * (size_t)ncell * SPBTE_MOM_STRIDE);
-  const size_t b_part = al256(sizeof(double) * 5 * (size_t)op->N * ncell);
+  const size_t b_part = al256(sizeof(double) * 10 * (size_t)op->N * ncell);
   const size_t b_fg = al256((size_t)nj * n * fg_bytes(form)), b_qh = al256(16 * n);
   const size_t b_xs = 16 * n * (size_t)nj, b_qp = (have && t.kind > 1) ? 16 * n * (size_t)t.kind : 0;
   const size_t b_x = al256(b_xs > b_qp ? b_xs : b_qp);
@@ -1029,7 +1057,7 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
     G = FG + 8 * n;
   }
   const scratch_set ss = own_scratch(op);
-  rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell, &ss);
+  rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell, &ss, 0, NULL, NULL);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(d_Qhat, op->d_Qh, 16 * n, hipMemcpyDeviceToDevice, op->s));
   return SPBTE_OK;
@@ -1107,10 +1135,12 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
   char *FG = (char *)op->d_FG;
   LCHK(spbte_launch_fill_pattern(FG, esz * n * 4 / (esz == 8 ? 4 : 8), esz == 8, op->s));
   spbte_wsum_args args = {NULL, FG, FG + esz * n, np == 2 ? FG + 2 * esz * n : NULL, np == 2 ? FG + 3 * esz * n : NULL,
-                          op->d_Qh, ncell, np};
+                          op->d_Qh, ncell, np, 0};
   const scratch_set ss = own_scratch(op);
   spbte_tile best = {0, 0, 0, 0};
   float best_ms = 1e30f;
+  const void *qsrc;
+  int nsl;
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
@@ -1122,7 +1152,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     /* warm-up (builds the table layout it needs); a tile whose warm-up alone
      * takes twice the best time so far is not timed further */
     HIPCHK(hipEventRecord(a, op->s));
-    rc = launch_tile(op, form, t, &args, &ss);
+    rc = launch_tile(op, form, t, &args, &ss, &qsrc, &nsl);
     HIPCHK(hipEventRecord(b, op->s));
     HIPCHK(hipEventSynchronize(b));
     if (rc) break;
@@ -1131,7 +1161,7 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
     if (warm > 2.0f * best_ms) continue;
     HIPCHK(hipEventRecord(a, op->s));
     const int reps = 2;
-    for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args, &ss);
+    for (int r = 0; r < reps && rc == SPBTE_OK; r++) rc = launch_tile(op, form, t, &args, &ss, &qsrc, &nsl);
     HIPCHK(hipEventRecord(b, op->s));
     HIPCHK(hipEventSynchronize(b));
     if (rc) break;

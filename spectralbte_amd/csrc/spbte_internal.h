@@ -38,6 +38,8 @@ typedef struct {
   const double *mom; /* SPBTE_MOM_STRIDE per cell, or NULL for mode 0 */
   int mode;
   void *dst;         /* complex output, double2 or float2 per element */
+  const double *part; /* instead of mom (FFT grids): the per-plane moment partial
+                         sums of spbte_launch_moment_partials, 5 N per cell */
 } spbte_xform_job;
 
 /* Weighted-sum kernel arguments.  F/G arrays hold ncell * N^3 complex values
@@ -48,6 +50,8 @@ typedef struct {
   void *Qh;
   int ncell;
   int nprod;          /* 1: ComputeQ, 2: fused maxPreserve */
+  int keep_slices;    /* split streaming tiles: leave the slices' partial sums in
+                         Qp for the consumer to add (Qh is then not written) */
 } spbte_wsum_args;
 
 /* Tile of a weighted-sum kernel.  kind 0: k_wsum.hip (lines = z-lines of output
@@ -70,8 +74,20 @@ int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform_job *jobs,
                            void *scratch, hipStream_t s);
 int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void *out, int ncell, int invert,
                                    void *scratch, hipStream_t s);
-int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, double *q, int ncell, void *scratch,
-                                hipStream_t s);
+/* Qh: nsl sets of ncell * N^3 double2, ncell * N^3 apart, whose sum is
+ * transformed (nsl > 1 on FFT grids only: the split weighted sum's slices; Qh
+ * may then alias scratch).  proj_part (FFT grids only, else NULL): also writes
+ * the one-species projection's per-plane partial sums of q (5 N per cell) for
+ * spbte_launch_project_apply. */
+int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell, void *scratch,
+                                double *proj_part, hipStream_t s);
+/* The first pass of spbte_launch_moments alone (part: 5 N per cell), whose
+ * partials FFT transform jobs read in place of the moments. */
+int spbte_launch_moment_partials(const spbte_grid_dev *g, const double *f, int ncell, double *part, hipStream_t s);
+/* The second pass of the one-species projection, on partials written by the
+ * inverse transform. */
+int spbte_launch_project_apply(const spbte_grid_dev *g, const double *masses, const double *LU, const int *pivot,
+                               double *Q, int ncell, const double *part, hipStream_t s);
 int spbte_launch_wsum(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, hipStream_t s);
 int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, const float *rowscale,
                              hipStream_t s);

@@ -336,7 +336,7 @@ static int backward(spbte_net *n, int B) {
 static int moments_and_features(spbte_net *n, const double *d_f, int B) {
   const spbte_grid_dev *g = spbte_op_grid(n->op);
   LCHK(spbte_launch_moments(g, d_f, n->mom, B, NULL, n->s));
-  spbte_xform_job jobs[2] = {{d_f, n->mom, 1, n->Fd}, {d_f, n->mom, 2, n->Fm}};
+  spbte_xform_job jobs[2] = {{d_f, n->mom, 1, n->Fd, NULL}, {d_f, n->mom, 2, n->Fm, NULL}};
   LCHK(spbte_launch_xform_fwd(g, jobs, 2, B, 0, n->xs, n->s));
   LCHK(spbte_net_features(n->N, n->Kb, B, n->Fd, n->Fm, n->mom, n->fscale, n->X, n->s));
   return SPBTE_OK;
@@ -516,7 +516,7 @@ int spbte_net_predict(spbte_net *n, const double *d_f, double *d_Q, int ncell, i
     CHK(moments_and_features(n, d_f + c0 * n3, B));
     CHK(forward(n, B));
     LCHK(spbte_net_scatter(n->N, n->Kb, B, n->Y, n->mom, n->tscale, n->Qh, n->s));
-    LCHK(spbte_launch_xform_inv_real(g, n->Qh, d_Q + c0 * n3, B, n->xs, n->s));
+    LCHK(spbte_launch_xform_inv_real(g, n->Qh, 1, d_Q + c0 * n3, B, n->xs, NULL, n->s));
   }
   if (conserve) CHK(spbte_conserve(n->op, d_Q, ncell));
   return SPBTE_OK;
