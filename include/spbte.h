@@ -165,9 +165,12 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
 /* Tiles of the weighted-sum kernels.  kind 0 is the tiled kernel (tr output rows
  * per lane, `lines` z-lines of output modes per workgroup, tb cells per
  * workgroup).  kind k >= 1 is the streaming kernel (tr rows per lane, `lines` =
- * waves per workgroup, tb cells per workgroup) with the summed x-planes split
- * into k slices whose partial sums are added afterwards (k > 1 gives small
- * batches more workgroups; k must divide N).  set_tile pins a tile for (form,
+ * waves per workgroup, tb cells per lane; N=32 with FP32 products also has
+ * lines = 16: four groups of 4 waves on the same rows, each group its own
+ * cell, so that one workgroup's W loads serve four cells) with the summed
+ * x-planes split into k slices whose partial sums are added afterwards (k > 1
+ * gives small batches more workgroups; k divides N, or is N times a divisor of
+ * N, cutting x-planes into y-line ranges).  set_tile pins a tile for (form,
  * maxpreserve); tr = 0 restores automatic choice.  autotune times every
  * instantiated tile on the given batch shape and pins the fastest for that
  * batch size; get_tile reports the tile a batch would use. */

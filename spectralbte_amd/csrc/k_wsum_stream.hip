@@ -99,8 +99,8 @@ __device__ __forceinline__ auto wget(const u32x4 &u, const u32x4 &u2, int j) {
 // with room to spare (N=16: 134 MB in FP64) keeps plain loads, so successive
 // evaluations find it there (the N=16 single-cell sum measured 29.0 us against
 // 30.8 us non-temporal).
-template <int N, int TR, int WAVES, int TB, int NP, int FORM, bool NTW>
-__global__ void __launch_bounds__(64 * WAVES)
+template <int N, int TR, int WAVES, int TB, int NP, int FORM, bool NTW, int CPW>
+__global__ void __launch_bounds__(64 * WAVES * CPW)
 k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
               const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
               double2 *__restrict__ Qh, int ncell, int csplit, double2 *__restrict__ Qp) {
@@ -115,9 +115,13 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   constexpr int BS = Gm::BS;                            // cz steps per loop body
 
   constexpr int LW = Gm::LW;
-  __shared__ __attribute__((aligned(16))) C sF[NP * TB * P * N * LW];  // [p][b][plane][sy][m]
+  // CPW cell groups per workgroup, WAVES waves each, on the same rows: their
+  // W loads hit the same lines at about the same time
+  constexpr int NT = 64 * WAVES * CPW;                  // threads per workgroup
+  constexpr int KB = CPW * TB;                          // cells per workgroup
+  __shared__ __attribute__((aligned(16))) C sF[NP * KB * P * N * LW];  // [p][k][plane][sy][m]
   constexpr bool SG = FORM != 0 || NP == 2;  // G by scalar loads (else staged in LDS)
-  __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * TB * N2];  // [p][b][cy][cz]
+  __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * KB * N2];  // [p][k][cy][cz]
 
   // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
   // dispatched round-robin over the 8 XCDs, so XCD x (bid mod 8) takes the x-th
@@ -125,7 +129,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   // triples, and the workgroups it runs back to back share a piece of W in its
   // L2.  With csplit > 1 the summed x-planes cx are split into csplit slices
   // (small batches: more workgroups), each writing partial sums to Qp.
-  const int ncg = (ncell + TB - 1) / TB;
+  const int ncg = (ncell + KB - 1) / KB;                // groups of KB cells
   const int total = NRT * csplit * ncg, per = (total + 7) >> 3;
   const int vid = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if ((blockIdx.x >> 3) >= per || vid >= total) return;  // the few padding workgroups of the last slice
@@ -137,10 +141,10 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   const int cx0 = csplit > N ? cs / ny : cs * (N / csplit);
   const int cx1 = csplit > N ? cx0 + 1 : cx0 + N / csplit;
   const int cy0 = csplit > N ? (cs - cx0 * ny) * (N / ny) : 0, cy1 = cy0 + N / ny;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WAVES, grp = (threadIdx.x >> 6) / WAVES;
   const size_t wg_row0 = (size_t)rt * RWG;
   const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
-  const size_t row0 = wg_row0 + (size_t)TR * threadIdx.x;
+  const size_t row0 = wg_row0 + (size_t)TR * (threadIdx.x - grp * 64 * WAVES);
   const int rx = (int)(row0 / N2), ry = (int)((row0 / N) % N), rz0 = (int)(row0 % N);
   const int pl = rx - rx0;                               // this lane's plane in LDS
   const int k0 = (rz0 + N / 2) % N;                      // window start at cz = 0 (non-negative)
@@ -151,7 +155,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   int cells[TB];
 #pragma unroll
   for (int b = 0; b < TB; ++b) {
-    const int c = cg * TB + b;
+    const int c = cg * KB + grp * TB + b;
     cells[b] = c < ncell ? c : ncell - 1;  // pad a ragged last group with a valid cell (not written)
   }
 
@@ -187,20 +191,22 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
     __syncthreads();
     // ---- stage the shifted F x-planes of every cell, product and lane plane,
     // each z-line extended to LW values (entry m = F[(m - BS) mod N])
-    for (int e = threadIdx.x; e < NP * TB * P * N * LW; e += 64 * WAVES) {
-      const int ln = e / LW, m = e - ln * LW;                 // line (p, b, q, sy), entry
-      const int p = ln / (TB * P * N), r1 = ln - p * (TB * P * N);
-      const int b = r1 / (P * N), r2 = r1 - b * (P * N);
+    for (int e = threadIdx.x; e < NP * KB * P * N * LW; e += NT) {
+      const int ln = e / LW, m = e - ln * LW;                 // line (p, k, q, sy), entry
+      const int p = ln / (KB * P * N), r1 = ln - p * (KB * P * N);
+      const int k = r1 / (P * N), r2 = r1 - k * (P * N);
       const int q = r2 / N, y = r2 - q * N;
       int sx = (rx0 + q) % N + N / 2 - cx;
       sx = sx < 0 ? sx + N : (sx > N - 1 ? sx - N : sx);
       const int z = m < BS ? m - BS + N : m - BS;
-      sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)cells[b] * n3 + (size_t)sx * N2 + (size_t)y * N + z];
+      const int c = cg * KB + k < ncell ? cg * KB + k : ncell - 1;
+      sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)c * n3 + (size_t)sx * N2 + (size_t)y * N + z];
     }
     if constexpr (!SG)
-      for (int e = threadIdx.x; e < NP * TB * N2; e += 64 * WAVES) {
-        const int p = e / (TB * N2), r1 = e - p * (TB * N2), b = r1 / N2, yz = r1 - b * N2;
-        sG[e] = Gp[p][(size_t)cells[b] * n3 + (size_t)cx * N2 + yz];
+      for (int e = threadIdx.x; e < NP * KB * N2; e += NT) {
+        const int p = e / (KB * N2), r1 = e - p * (KB * N2), k = r1 / N2, yz = r1 - k * N2;
+        const int c = cg * KB + k < ncell ? cg * KB + k : ncell - 1;
+        sG[e] = Gp[p][(size_t)c * n3 + (size_t)cx * N2 + yz];
       }
     __syncthreads();
 
@@ -211,7 +217,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * TB + b) * P + pl) * N + sy) * LW;
+        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * KB + grp * TB + b) * P + pl) * N + sy) * LW;
       // circular window: at step cz, element t (row rz0 + t) is buf[(t - cz) mod TR]
       // and holds F_line[(k0 + t - cz) mod N] = line[BS + (k0 + t - cz) mod N]
       C buf[NP][TB][TR];
@@ -322,7 +328,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 #pragma unroll
           for (int b = 0; b < TB; ++b)
             gz[p][b] = SG ? Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS
-                          : sG + (size_t)(p * TB + b) * N2 + (size_t)cy * N + it * BS;
+                          : sG + (size_t)(p * KB + grp * TB + b) * N2 + (size_t)cy * N + it * BS;
         // 8-step bodies fold every fourth body (32-term partial sums), longer
         // bodies every 8 steps; the last body of a line always folds
         fold_body = (BS >= 16) || ((it & 3) == 3) || (it == N / BS - 1);
@@ -339,7 +345,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 
 #pragma unroll
   for (int b = 0; b < TB; ++b) {
-    const int cell = cg * TB + b;
+    const int cell = cg * KB + grp * TB + b;
     if (cell >= ncell) continue;
 #pragma unroll
     for (int t = 0; t < TR; ++t) {
@@ -369,22 +375,25 @@ __global__ void __launch_bounds__(256) k_slice_sum(size_t n, int csplit, const d
   }
 }
 
-template <int N, int TR, int WAVES, int TB, int NP, int FORM>
+template <int N, int TR, int WAVES, int TB, int NP, int FORM, int CPW>
 int launch_s(const spbte_wsum_args *a, const void *Wp, const float *rowscale, int csplit, void *Qp, hipStream_t s) {
   using Gm = SGeom<N, TR, WAVES>;
-  const int ncg = (a->ncell + TB - 1) / TB;
+  const int ncg = (a->ncell + TB * CPW - 1) / (TB * CPW);
   const int nrt = (int)((size_t)N * N * N / Gm::RWG);
   const int per = (nrt * csplit * ncg + 7) / 8;  // workgroups per XCD slice
   constexpr size_t n3 = (size_t)N * N * N, kCacheResident = (size_t)160 << 20;
   constexpr bool resident = n3 * n3 * (FORM == 0 ? 8 : 4) <= kCacheResident;
-  if (ncg == 1 && !resident)  // every W element read once: non-temporal loads
-    hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, true>), dim3(8 * per), dim3(64 * WAVES), 0, s,
-                       (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
-                       (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
-                       (double2 *)Qp);
-  else
-    hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, false>), dim3(8 * per), dim3(64 * WAVES), 0, s,
-                       (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
+  bool nt = false;
+  if constexpr (CPW == 1) nt = ncg == 1 && !resident;  // every W element read once: non-temporal loads
+  if constexpr (CPW == 1)
+    if (nt)
+      hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, true, 1>), dim3(8 * per), dim3(64 * WAVES), 0, s,
+                         (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
+                         (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
+                         (double2 *)Qp);
+  if (!nt)
+    hipLaunchKernelGGL((k_wsum_stream<N, TR, WAVES, TB, NP, FORM, false, CPW>), dim3(8 * per), dim3(64 * WAVES * CPW), 0,
+                       s, (const uint4 *)Wp, rowscale, (const double2 *)a->F0, (const double2 *)a->G0,
                        (const double2 *)a->F1, (const double2 *)a->G1, (double2 *)a->Qh, a->ncell, csplit,
                        (double2 *)Qp);
   if (csplit > 1 && !a->keep_slices) {
@@ -401,7 +410,7 @@ struct SEntry {
   launch_fn fn;
 };
 
-#define S(N, TR, WV, TB, NP, F) {N, TR, WV, TB, NP, F, &launch_s<N, TR, WV, TB, NP, F>}
+#define S(N, TR, WV, TB, NP, F) {N, TR, WV, TB, NP, F, &launch_s<N, TR, WV, TB, NP, F, 1>}
 #define S3(N, TR, WV, TB, NP) S(N, TR, WV, TB, NP, 0), S(N, TR, WV, TB, NP, 1), S(N, TR, WV, TB, NP, 2)
 #define S4(N, TR, WV, TB, NP) S3(N, TR, WV, TB, NP), S(N, TR, WV, TB, NP, 3)
 
@@ -411,6 +420,9 @@ const SEntry kSTable[] = {
     S3(32, 4, 4, 1, 1), S3(32, 8, 2, 1, 2), S4(32, 4, 4, 2, 2), S4(32, 4, 4, 1, 2),
     // small batches: more, smaller r-tiles (half and quarter x-planes)
     S3(32, 4, 2, 1, 1), S3(32, 4, 2, 1, 2), S3(32, 4, 1, 1, 1), S3(32, 4, 1, 2, 1),
+    // FP32 products, maxPreserve: four 4-wave groups of the same rows, one cell
+    // each, per workgroup ("16 waves")
+    {32, 4, 16, 1, 2, 3, &launch_s<32, 4, 4, 1, 2, 3, 4>},
     // N = 24 (rows per workgroup must divide 24^3): TR=3 x 3 waves = one x-plane,
     // TR=6 x 3 waves = 2 planes, TR=4 x 2 waves and TR=8 x 1 wave straddle planes
     S4(24, 3, 3, 1, 1), S4(24, 3, 3, 2, 1), S4(24, 3, 3, 1, 2), S3(24, 6, 3, 1, 1), S4(24, 6, 3, 1, 2),

@@ -714,7 +714,7 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * 41.6 us against 48.3 with 16 whole planes and 99 with 8 slices of 4 rows;
  * 64 slices took 56.8 us, the F staging outweighing the extra waves). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
-  static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}},
+  static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 16, 1, 1}, {4, 4, 1, 1}},
                          s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s24ma[] = {{3, 3, 2, 1}, {3, 3, 1, 1}};
@@ -725,7 +725,7 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   if (N == 32) {
     if (nprod == 1) { c = s32; nc = 3; }
     else if (form == SPBTE_FORM_FP64) { c = s32m64; nc = 1; }
-    else { c = s32m; nc = 1; }
+    else { c = s32m; nc = 2; }
   } else if (N == 24) {
     if (nprod == 2 && form == SPBTE_FORM_FP32A) { c = s24ma; nc = 2; }
     else { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
@@ -738,6 +738,8 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   for (int i = 0; i < nc; i++) {
     if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
     if (c[i].tb > ncell && i + 1 < nc) continue;
+    /* a 16-wave tile (four cells per workgroup) only where it fills the chip */
+    if (c[i].lines == 16 && stream_wgs(N, ncell, &c[i]) < 2048 && i + 1 < nc) continue;
     *t = c[i];
     const long wgs = stream_wgs(N, ncell, t);
     int k = 1;
@@ -1112,7 +1114,7 @@ static const spbte_tile k_cand[] = {
     {4, 1, 1, 0}, {4, 2, 1, 0}, {4, 2, 2, 0}, {4, 4, 2, 0}, {4, 2, 4, 0}, {4, 4, 4, 0}, {8, 4, 2, 0}, {8, 8, 2, 0},
     {8, 4, 4, 0}, {6, 8, 1, 0}, {6, 8, 2, 0}, {6, 8, 4, 0}, {4, 8, 2, 0}, {4, 8, 4, 0}, {4, 4, 1, 0}, {2, 8, 2, 0},
     {8, 2, 1, 1}, {8, 2, 2, 1}, {4, 4, 2, 1}, {4, 2, 2, 1}, {4, 4, 4, 1}, {4, 4, 1, 1}, {6, 3, 1, 1}, {3, 3, 1, 1},
-    {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1},
+    {3, 3, 2, 1}, {4, 2, 1, 1}, {8, 1, 1, 1}, {4, 1, 1, 1}, {4, 1, 2, 1}, {4, 9, 1, 1}, {4, 16, 1, 1},
     /* streaming with the summed x-planes split in 2-8 slices, for small batches */
     {4, 4, 1, 2}, {4, 4, 1, 4}, {4, 4, 1, 8}, {4, 4, 2, 2}, {4, 4, 2, 4}, {4, 4, 2, 8}, {4, 2, 1, 4}, {4, 2, 1, 8},
     {4, 2, 2, 4}, {8, 2, 1, 4}, {8, 2, 1, 8}, {4, 2, 1, 2}, {4, 9, 1, 2}, {4, 9, 1, 4},

@@ -136,7 +136,9 @@ def test_n32_tiles_agree(spb, orc):
 def test_n32_fp32_products_streaming(spb, orc):
     """The FP32-product form on the streaming kernel (every instantiated N=32
     tile, one and two products, whole and split x-plane sums) against the FP64
-    sum: within 1e-6 relative L2 per problem, like the other reduced forms."""
+    sum: within 1e-6 relative L2 per problem, like the other reduced forms.  The
+    16-wave tile (four cells per workgroup, here 5 problems: one group padded)
+    runs each lane's sum in the same order as the 4-wave tile: bit-identical."""
     N, Lv = 32, 6.0
     v, eta = orc.grid_0d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
@@ -147,16 +149,20 @@ def test_n32_fp32_products_streaming(spb, orc):
     for mp in (False, True):
         op.set_tile("fp64", mp, 4, 4, 2, 0)
         ref = op.compute_q(f, maxpreserve=mp)
-        for t in [(4, 4, 1, 1), (4, 4, 2, 1), (4, 4, 4, 1), (4, 4, 1, 4), (4, 4, 2, 2)]:
+        got = {}
+        for t in [(4, 4, 1, 1), (4, 4, 2, 1), (4, 4, 4, 1), (4, 4, 1, 4), (4, 4, 2, 2), (4, 16, 1, 1)]:
             try:
                 op.set_tile("fp32a", mp, *t)
             except spb.SpbteError:
                 continue  # not instantiated for this product count
             Q = op.compute_q(f, form="fp32a", maxpreserve=mp)
+            got[t] = Q
             for b in range(len(f)):
                 assert rel_l2(Q[b], ref[b]) < 1e-6, (mp, t, b)
             ran += 1
-    assert ran >= 5
+        if mp:
+            assert np.array_equal(got[(4, 16, 1, 1)], got[(4, 4, 1, 1)])
+    assert ran >= 6
     op.close()
 
 
