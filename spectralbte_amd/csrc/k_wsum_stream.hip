@@ -141,7 +141,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   const int cx0 = csplit > N ? cs / ny : cs * (N / csplit);
   const int cx1 = csplit > N ? cx0 + 1 : cx0 + N / csplit;
   const int cy0 = csplit > N ? (cs - cx0 * ny) * (N / ny) : 0, cy1 = cy0 + N / ny;
-  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WAVES, grp = (threadIdx.x >> 6) / WAVES;
+  // wave and cell group are wave-uniform: say so, so that the cell's G lines
+  // stay scalar loads
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) % WAVES);
+  const int grp = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) / WAVES);
   const size_t wg_row0 = (size_t)rt * RWG;
   const int rx0 = (int)(wg_row0 / N2);                  // first x-plane of the workgroup
   const size_t row0 = wg_row0 + (size_t)TR * (threadIdx.x - grp * 64 * WAVES);

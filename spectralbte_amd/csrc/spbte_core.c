@@ -706,7 +706,10 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * waves; the one-product (ComputeQ) sum at N=32 four cells per workgroup when
  * the batch has them; the FP32-product maxPreserve sum at N=24 takes 3 rows x 3 waves with
  * two problems per lane (31.7 ms for 512 problems against 44.2 ms on the tiled
- * kernel).  Unsupported candidates are skipped.
+ * kernel).  The 16-wave tile of the FP32-product maxPreserve sum at N=32 (four
+ * cells per workgroup: half the W refetch) is left to the autotuner: on 512
+ * problems it ran 5.8% faster than the 4-wave tile on one MI355X and 4.7%
+ * slower on another.  Unsupported candidates are skipped.
  * Batches too small to give 2048 workgroups split the summed x-planes into up
  * to 8 slices; at N=16, where one to four cells take one row per lane, into
  * as many as give 2048 workgroups, cutting x-planes into y-line ranges if
@@ -714,7 +717,7 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * 41.6 us against 48.3 with 16 whole planes and 99 with 8 slices of 4 rows;
  * 64 slices took 56.8 us, the F staging outweighing the extra waves). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
-  static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 16, 1, 1}, {4, 4, 1, 1}},
+  static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}},
                          s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s24ma[] = {{3, 3, 2, 1}, {3, 3, 1, 1}};
@@ -725,7 +728,7 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   if (N == 32) {
     if (nprod == 1) { c = s32; nc = 3; }
     else if (form == SPBTE_FORM_FP64) { c = s32m64; nc = 1; }
-    else { c = s32m; nc = 2; }
+    else { c = s32m; nc = 1; }
   } else if (N == 24) {
     if (nprod == 2 && form == SPBTE_FORM_FP32A) { c = s24ma; nc = 2; }
     else { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
@@ -738,8 +741,6 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
   for (int i = 0; i < nc; i++) {
     if (!spbte_stream_tile_supported(N, form, nprod, c[i].tr, c[i].lines, c[i].tb)) continue;
     if (c[i].tb > ncell && i + 1 < nc) continue;
-    /* a 16-wave tile (four cells per workgroup) only where it fills the chip */
-    if (c[i].lines == 16 && stream_wgs(N, ncell, &c[i]) < 2048 && i + 1 < nc) continue;
     *t = c[i];
     const long wgs = stream_wgs(N, ncell, t);
     int k = 1;
