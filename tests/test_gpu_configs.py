@@ -85,9 +85,11 @@ def test_headline_single_problem_nontemporal(spb, orc, ensemble32):
         assert rel_l2(one32, q64[b]) < 1e-6, (b, rel_l2(one32, q64[b]))
 
 
-def test_n16_fp32a_two_cell_groups_nontemporal(spb, orc):
-    """N=16, 1 and 2 cells on the streaming tile with 2 cells per workgroup
-    (one cell group: non-temporal W loads), one and two products, against FP64."""
+def test_n16_fp32a_single_cell_group(spb, orc):
+    """N=16, 1 and 2 cells on the streaming tile with 2 cells per lane (one cell
+    group; the 67 MB FP32 table fits in the Infinity Cache, so W keeps plain
+    loads, and the N=32 single-problem test covers the non-temporal
+    instantiations), one and two products, against FP64."""
     from spectralbte_amd import workloads
     N, Lv = 16, 5.0
     v, eta = workloads.grid_0d(N, Lv)
