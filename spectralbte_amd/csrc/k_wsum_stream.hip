@@ -99,8 +99,15 @@ __device__ __forceinline__ auto wget(const u32x4 &u, const u32x4 &u2, int j) {
 // with room to spare (N=16: 134 MB in FP64) keeps plain loads, so successive
 // evaluations find it there (the N=16 single-cell sum measured 29.0 us against
 // 30.8 us non-temporal).
+// Four problems per lane with FP64 arithmetic: G by scalar loads and at most
+// 256 VGPRs, so that two workgroups (80 KiB of LDS each) share a CU and every
+// SIMD runs two waves (ComputeQ on 256 shock cells: 65.5 ms against 73.0 ms
+// with G staged in LDS at one wave per SIMD).
+template <int TB, int FORM>
+constexpr int kWavesPerEU = (TB >= 4 && FORM != 3) ? 2 : 1;
+
 template <int N, int TR, int WAVES, int TB, int NP, int FORM, bool NTW, int CPW>
-__global__ void __launch_bounds__(64 * WAVES * CPW)
+__global__ void __launch_bounds__(64 * WAVES * CPW) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<TB, FORM>)))
 k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
               const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
               double2 *__restrict__ Qh, int ncell, int csplit, double2 *__restrict__ Qp) {
@@ -120,7 +127,7 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   constexpr int NT = 64 * WAVES * CPW;                  // threads per workgroup
   constexpr int KB = CPW * TB;                          // cells per workgroup
   __shared__ __attribute__((aligned(16))) C sF[NP * KB * P * N * LW];  // [p][k][plane][sy][m]
-  constexpr bool SG = FORM != 0 || NP == 2;  // G by scalar loads (else staged in LDS)
+  constexpr bool SG = FORM != 0 || NP == 2 || TB >= 4;  // G by scalar loads (else staged in LDS)
   __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * KB * N2];  // [p][k][cy][cz]
 
   // ---- workgroup -> (r-tile, cx slice, cell group), XCD-aware: workgroups are
