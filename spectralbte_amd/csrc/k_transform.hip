@@ -167,7 +167,11 @@ __global__ void __launch_bounds__(XT) k_xform_b(int N, int ncell, const double2 
       acc.x *= pre;
       acc.y *= pre;
       if (OUT == 0) ((double2 *)dst)[o] = acc;
-      else ((float2 *)dst)[o] = make_float2((float)acc.x, (float)acc.y);
+      else {
+        const float2 h = make_float2((float)acc.x, (float)acc.y);
+        ((float2 *)dst)[o] = h;
+        ((float *)((float2 *)dst + ncell * n3))[o] = h.x + h.y;  // the parts' sums after the values
+      }
     }
   }
 }
@@ -414,8 +418,13 @@ __global__ void __launch_bounds__(XT) k_fft_b(int ncell, const double2 *__restri
     f.x *= pre;
     f.y *= pre;
     if (OUT == 0) ((double2 *)dst)[o] = f;
-    else if (OUT == 1) ((float2 *)dst)[o] = make_float2((float)f.x, (float)f.y);
-    else {
+    else if (OUT == 1) {
+      // FP32 operands: the complex values, then (after ncell N^3 of them) the
+      // sums of their parts, which the FP32-product sum takes as G's a + b
+      const float2 h = make_float2((float)f.x, (float)f.y);
+      ((float2 *)dst)[o] = h;
+      ((float *)((float2 *)dst + ncell * n3))[o] = h.x + h.y;
+    } else {
       ((double *)dst)[o] = f.x;
       if (ppart) {
         const double p = w[a] * w[b] * w[c] * pm * f.x;

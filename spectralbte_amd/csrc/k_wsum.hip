@@ -466,9 +466,14 @@ extern "C" int spbte_launch_convert_weights(const double *W64, void *dst, int fo
 }
 
 namespace {
+// FP32 operands: n complex values, then the n sums of their parts.
 __global__ void k_c64_to_c32(const double2 *__restrict__ in, float2 *__restrict__ out, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+  float *sums = (float *)(out + n);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 h = make_float2((float)in[i].x, (float)in[i].y);
+    out[i] = h;
+    sums[i] = h.x + h.y;
+  }
 }
 }  // namespace
 

@@ -102,12 +102,14 @@ __device__ __forceinline__ auto wget(const u32x4 &u, const u32x4 &u2, int j) {
 // Four problems per lane with FP64 arithmetic: G by scalar loads and at most
 // 256 VGPRs, so that two workgroups (80 KiB of LDS each) share a CU and every
 // SIMD runs two waves (ComputeQ on 256 shock cells: 65.5 ms against 73.0 ms
-// with G staged in LDS at one wave per SIMD).
-template <int TB, int FORM>
+// with G staged in LDS at one wave per SIMD).  (Capping the FP32-product tile
+// of 4 rows x 4 waves at 128 VGPRs for a fourth wave per SIMD made it 1.8x
+// slower: it runs at 141 VGPRs, three waves per SIMD.)
+template <int TR, int WAVES, int TB, int FORM>
 constexpr int kWavesPerEU = (TB >= 4 && FORM != 3) ? 2 : 1;
 
 template <int N, int TR, int WAVES, int TB, int NP, int FORM, bool NTW, int CPW>
-__global__ void __launch_bounds__(64 * WAVES * CPW) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<TB, FORM>)))
+__global__ void __launch_bounds__(64 * WAVES * CPW) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<TR, WAVES, TB, FORM>)))
 k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, const double2 *__restrict__ F0,
               const double2 *__restrict__ G0, const double2 *__restrict__ F1, const double2 *__restrict__ G1,
               double2 *__restrict__ Qh, int ncell, int csplit, double2 *__restrict__ Qp) {
@@ -127,6 +129,16 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   constexpr int NT = 64 * WAVES * CPW;                  // threads per workgroup
   constexpr int KB = CPW * TB;                          // cells per workgroup
   __shared__ __attribute__((aligned(16))) C sF[NP * KB * P * N * LW];  // [p][k][plane][sy][m]
+  // FP32 products, two per pair (maxPreserve): each complex product
+  // (a + ib)(c + id) with G = a + ib wave-uniform takes Gauss's three
+  // multiplications, re = c (a + b) - b (c + d) and im = c (a + b) + a (d - c),
+  // and the two products share one MUL + FMA for the c (a + b) terms: 8
+  // instructions per pair with w instead of 10.  The staged F entry holds
+  // (c, d - c) in sF and c + d in sP; a + b comes from the sums stored after
+  // G's values.  (With one product the saving is 1 of 6, and the extra window
+  // registers cost more than it: ComputeQ keeps the plain form.)
+  constexpr bool GS = FORM == 3 && NP == 2;
+  __shared__ float sP[GS ? NP * KB * P * N * LW : 1];
   constexpr bool SG = FORM != 0 || NP == 2 || TB >= 4;  // G by scalar loads (else staged in LDS)
   __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * KB * N2];  // [p][k][cy][cz]
 
@@ -163,6 +175,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
 
   const C *Fp[2] = {(const C *)F0, (const C *)F1};
   const C *Gp[2] = {(const C *)G0, (const C *)G1};
+  const float *Gsum[2] = {nullptr, nullptr};
+  if constexpr (GS) {
+    Gsum[0] = (const float *)(Gp[0] + (size_t)ncell * n3);
+    if constexpr (NP == 2) Gsum[1] = (const float *)(Gp[1] + (size_t)ncell * n3);
+  }
   int cells[TB];
 #pragma unroll
   for (int b = 0; b < TB; ++b) {
@@ -211,7 +228,13 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       sx = sx < 0 ? sx + N : (sx > N - 1 ? sx - N : sx);
       const int z = m < BS ? m - BS + N : m - BS;
       const int c = cg * KB + k < ncell ? cg * KB + k : ncell - 1;
-      sF[ln * LW + Gm::pos(m)] = Fp[p][(size_t)c * n3 + (size_t)sx * N2 + (size_t)y * N + z];
+      const C val = Fp[p][(size_t)c * n3 + (size_t)sx * N2 + (size_t)y * N + z];
+      if constexpr (GS) {
+        sF[ln * LW + Gm::pos(m)] = make_float2(val.x, val.y - val.x);
+        sP[ln * LW + Gm::pos(m)] = val.x + val.y;
+      } else {
+        sF[ln * LW + Gm::pos(m)] = val;
+      }
     }
     if constexpr (!SG)
       for (int e = threadIdx.x; e < NP * KB * N2; e += NT) {
@@ -225,26 +248,37 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       int sy = ry + N / 2 - cy;
       sy = sy < 0 ? sy + N : (sy > N - 1 ? sy - N : sy);
       const C *line[NP][TB];
+      const float *lineP[NP][TB];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int b = 0; b < TB; ++b) line[p][b] = sF + ((size_t)((p * KB + grp * TB + b) * P + pl) * N + sy) * LW;
+        for (int b = 0; b < TB; ++b) {
+          const size_t off = ((size_t)((p * KB + grp * TB + b) * P + pl) * N + sy) * LW;
+          line[p][b] = sF + off;
+          lineP[p][b] = sP + (GS ? off : 0);
+        }
       // circular window: at step cz, element t (row rz0 + t) is buf[(t - cz) mod TR]
       // and holds F_line[(k0 + t - cz) mod N] = line[BS + (k0 + t - cz) mod N]
+      // (bufP: its c + d with FP32 products)
       C buf[NP][TB][TR];
+      float bufP[NP][GS ? TB : 1][GS ? TR : 1];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int b = 0; b < TB; ++b)
 #pragma unroll
-          for (int t = 0; t < TR; ++t) buf[p][b][t] = line[p][b][Gm::pos(BS + ((k0 + t) % N))];
+          for (int t = 0; t < TR; ++t) {
+            buf[p][b][t] = line[p][b][Gm::pos(BS + ((k0 + t) % N))];
+            if constexpr (GS) bufP[p][b][t] = lineP[p][b][Gm::pos(BS + ((k0 + t) % N))];
+          }
       const int c4base = (cx * N + cy) * (N / 4);
       bool fold_body = true;
       // One 4-step chunk at body offset jb (a compile-time constant after
       // unrolling): wc holds its weights, wn receives the next chunk's.  `fl`
       // points at the refill entry of body step 0: line + BS + (k0 - 1 - cz_body) mod N.
       auto chunk = [&](u32x4 (&wc)[TR][UPW], u32x4 (&wn)[TR][UPW], const C *const (&fl)[NP][TB],
-                       const C *const (&gz)[NP][TB], int jb, int c4n) __attribute__((always_inline)) {
+                       const float *const (&flP)[NP][TB], const C *const (&gz)[NP][TB],
+                       const float *const (&gzs)[NP][TB], int jb, int c4n) __attribute__((always_inline)) {
         // The prefetch of the step after the last is clamped to the last chunk
         // (loaded and not used) rather than skipped: a conditional load would
         // make the compiler copy both register sets at every chunk.  The
@@ -269,24 +303,40 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
             const auto w = wget<FORM>(wc[t][0], wc[t][UPW - 1], jj);
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
-              // complex products as explicit FMA chains: 4 ops per product, 2 for w
-              const C g0 = gz[0][b][j], f0 = buf[0][b][sl];
-              auto pr = g0.x * f0.x, pi = g0.x * f0.y;
-              pr = fma(-g0.y, f0.y, pr);
-              pi = fma(g0.y, f0.x, pi);
-              if constexpr (NP == 2) {
-                const C g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
-                pr = fma(g1.x, f1.x, pr);
-                pi = fma(g1.x, f1.y, pi);
-                pr = fma(-g1.y, f1.y, pr);
-                pi = fma(g1.y, f1.x, pi);
-              }
-              if constexpr (FORM == 3) {
-                a32[b][t].x = fmaf(w, pr, a32[b][t].x);
-                a32[b][t].y = fmaf(w, pi, a32[b][t].y);
+              if constexpr (GS) {
+                // Gauss's form: 1 + 2 ops per product (the shared c (a + b)
+                // terms of both products in one MUL + FMA), 2 for w
+                const C g0 = gz[0][b][j], f0 = buf[0][b][sl];
+                float k1 = f0.x * gzs[0][b][j];
+                if constexpr (NP == 2) k1 = fmaf(buf[NP - 1][b][sl].x, gzs[NP - 1][b][j], k1);
+                float qr = fmaf(-g0.y, bufP[0][b][sl], k1), qi = fmaf(g0.x, f0.y, k1);
+                if constexpr (NP == 2) {
+                  const C g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
+                  qr = fmaf(-g1.y, bufP[NP - 1][b][sl], qr);
+                  qi = fmaf(g1.x, f1.y, qi);
+                }
+                a32[b][t].x = fmaf(w, qr, a32[b][t].x);
+                a32[b][t].y = fmaf(w, qi, a32[b][t].y);
               } else {
-                acc[b][t].x = fma(w, pr, acc[b][t].x);
-                acc[b][t].y = fma(w, pi, acc[b][t].y);
+                // complex products as explicit FMA chains: 4 ops per product, 2 for w
+                const C g0 = gz[0][b][j], f0 = buf[0][b][sl];
+                auto pr = g0.x * f0.x, pi = g0.x * f0.y;
+                pr = fma(-g0.y, f0.y, pr);
+                pi = fma(g0.y, f0.x, pi);
+                if constexpr (NP == 2) {
+                  const C g1 = gz[NP - 1][b][j], f1 = buf[NP - 1][b][sl];
+                  pr = fma(g1.x, f1.x, pr);
+                  pi = fma(g1.x, f1.y, pi);
+                  pr = fma(-g1.y, f1.y, pr);
+                  pi = fma(g1.y, f1.x, pi);
+                }
+                if constexpr (FORM == 3) {
+                  a32[b][t].x = fmaf(w, pr, a32[b][t].x);
+                  a32[b][t].y = fmaf(w, pi, a32[b][t].y);
+                } else {
+                  acc[b][t].x = fma(w, pr, acc[b][t].x);
+                  acc[b][t].y = fma(w, pi, acc[b][t].y);
+                }
               }
             }
             if (t == TR - 1) {
@@ -297,9 +347,9 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
                 for (int b = 0; b < TB; ++b) {
                   // entry BS + base - j: with transposed lines BS + base = G BG + G - 1
                   // and fl points at slot BG of sub-row 0
-                  if constexpr (Gm::TL)
-                    buf[p][b][slnew] = fl[p][b][(Gm::G - 1 - j % Gm::G) * Gm::LS - j / Gm::G];
-                  else buf[p][b][slnew] = fl[p][b][-j];
+                  const int ro = Gm::TL ? (Gm::G - 1 - j % Gm::G) * Gm::LS - j / Gm::G : -j;
+                  buf[p][b][slnew] = fl[p][b][ro];
+                  if constexpr (GS) bufP[p][b][slnew] = flP[p][b][ro];
                 }
             }
           }
@@ -326,28 +376,35 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
       for (int it = 0; it < N / BS; ++it) {
         base = base >= (unsigned)N ? base - N : base;
         const C *fl[NP][TB];
+        const float *flP[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-          for (int b = 0; b < TB; ++b)
-            fl[p][b] = Gm::TL ? line[p][b] + ((unsigned)BS + base) / (unsigned)Gm::G : line[p][b] + BS + base;
+          for (int b = 0; b < TB; ++b) {
+            const unsigned o = Gm::TL ? ((unsigned)BS + base) / (unsigned)Gm::G : BS + base;
+            fl[p][b] = line[p][b] + o;
+            flP[p][b] = lineP[p][b] + (GS ? o : 0);
+          }
         // G_b[cx][cy][it BS ...] of product p, cell b: uniform addresses (scalar
         // loads), or the staged x-plane in LDS
         const C *gz[NP][TB];
+        const float *gzs[NP][TB];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-          for (int b = 0; b < TB; ++b)
-            gz[p][b] = SG ? Gp[p] + (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS
-                          : sG + (size_t)(p * KB + grp * TB + b) * N2 + (size_t)cy * N + it * BS;
+          for (int b = 0; b < TB; ++b) {
+            const size_t go = (size_t)cells[b] * n3 + (size_t)cx * N2 + (size_t)cy * N + it * BS;
+            gz[p][b] = SG ? Gp[p] + go : sG + (size_t)(p * KB + grp * TB + b) * N2 + (size_t)cy * N + it * BS;
+            gzs[p][b] = GS ? Gsum[p] + go : nullptr;
+          }
         // 8-step bodies fold every fourth body (32-term partial sums), longer
         // bodies every 8 steps; the last body of a line always folds
         fold_body = (BS >= 16) || ((it & 3) == 3) || (it == N / BS - 1);
 #pragma unroll
         for (int k = 0; k < BS / 4; k += 2) {
           const int c4 = c4base + it * (BS / 4) + k;
-          chunk(wA, wB, fl, gz, 4 * k, c4 + 1);
-          chunk(wB, wA, fl, gz, 4 * k + 4, c4 + 2);
+          chunk(wA, wB, fl, flP, gz, gzs, 4 * k, c4 + 1);
+          chunk(wB, wA, fl, flP, gz, gzs, 4 * k + 4, c4 + 2);
         }
         base = base - BS + N;
       }

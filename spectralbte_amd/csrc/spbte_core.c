@@ -79,7 +79,7 @@ struct spbte_op {
   int cap;
   double *d_mom;     /* 2 * cap * SPBTE_MOM_STRIDE */
   void *d_xs;        /* transform scratch: MAXJOBS * cap * n3 double2 */
-  void *d_FG;        /* MAXJOBS * cap * n3 double2 (FP32 forms use half) */
+  void *d_FG;        /* MAXJOBS * cap * n3 double2 (FP32 operands with their part sums use 12 bytes a value) */
   void *d_Qh;        /* cap * n3 double2 */
   double *d_part;    /* 10 * N * cap: per-plane partial sums of the moments (f's, g's) and the projection */
   /* tiles pinned per (form, nprod-1); tr == 0 -> automatic */
@@ -610,6 +610,12 @@ static int storage_of(int form) { return form == SPBTE_FORM_FP32A ? SPBTE_FORM_F
 /* F/G element size of a form: FP32 complex only when the arithmetic is FP32. */
 static size_t fg_bytes(int form) { return form == SPBTE_FORM_FP32A ? 8 : 16; }
 
+/* Bytes of one transformed operand of n values: FP64 complex, or FP32 complex
+ * followed by the n sums of their parts (the FP32-product sum's a + b). */
+static size_t fg_stride(int form, size_t n) {
+  return form == SPBTE_FORM_FP32A ? ((12 * n + 255) & ~(size_t)255) : 16 * n;
+}
+
 static int weights_form(spbte_op *op, int form, int build, const void **W, const float **rowscale) {
   if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
   *rowscale = NULL;
@@ -861,10 +867,10 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
   const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
   const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
   const size_t n = (size_t)ncell * op->n3;
-  const size_t esz = fg_bytes(form);
+  const size_t stride = fg_stride(form, n);
   char *FG = (char *)ss->FG;
   void *B[MAXJOBS];
-  for (int q = 0; q < MAXJOBS; q++) B[q] = FG + (size_t)q * n * esz;
+  for (int q = 0; q < MAXJOBS; q++) B[q] = FG + (size_t)q * stride;
   spbte_xform_job jobs[MAXJOBS];
   int nj = 0;
   const void *F0, *G0, *F1 = NULL, *G1 = NULL;
@@ -972,7 +978,7 @@ int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, 
   const size_t n = (size_t)ncell * op->n3;
   const size_t b_mom = al256(sizeof(double) * 2 * (size_t)ncell * SPBTE_MOM_STRIDE);
   const size_t b_part = al256(sizeof(double) * 10 * (size_t)op->N * ncell);
-  const size_t b_fg = al256((size_t)nj * n * fg_bytes(form)), b_qh = al256(16 * n);
+  const size_t b_fg = al256((size_t)nj * fg_stride(form, n)), b_qh = al256(16 * n);
   const size_t b_xs = 16 * n * (size_t)nj, b_qp = (have && t.kind > 1) ? 16 * n * (size_t)t.kind : 0;
   const size_t b_x = al256(b_xs > b_qp ? b_xs : b_qp);
   const size_t need = b_mom + b_part + b_fg + b_qh + b_x;
@@ -1054,10 +1060,11 @@ int spbte_qhat(spbte_op *op, const double *d_Fhat, const double *d_Ghat, double 
   const size_t n = (size_t)ncell * op->n3;
   if (fg_bytes(form) == 8) {
     char *FG = (char *)op->d_FG;
+    const size_t stride = fg_stride(form, n);
     LCHK(spbte_launch_c64_to_c32(d_Fhat, FG, n, op->s));
-    LCHK(spbte_launch_c64_to_c32(d_Ghat, FG + 8 * n, n, op->s));
+    LCHK(spbte_launch_c64_to_c32(d_Ghat, FG + stride, n, op->s));
     F = FG;
-    G = FG + 8 * n;
+    G = FG + stride;
   }
   const scratch_set ss = own_scratch(op);
   rc = run_wsum(op, form, 1, F, G, NULL, NULL, ncell, &ss, 0, NULL, NULL);
@@ -1134,10 +1141,11 @@ int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind
    * FP64 load the clock depends on the operands, and zeros ranked the tiles
    * differently from real transforms (12% apart on a 256-cell shock batch) */
   const size_t n = (size_t)ncell * op->n3;
-  const size_t esz = fg_bytes(form);
+  const int single = fg_bytes(form) == 8;
+  const size_t stride = fg_stride(form, n);
   char *FG = (char *)op->d_FG;
-  LCHK(spbte_launch_fill_pattern(FG, esz * n * 4 / (esz == 8 ? 4 : 8), esz == 8, op->s));
-  spbte_wsum_args args = {NULL, FG, FG + esz * n, np == 2 ? FG + 2 * esz * n : NULL, np == 2 ? FG + 3 * esz * n : NULL,
+  LCHK(spbte_launch_fill_pattern(FG, 4 * stride / (single ? 4 : 8), single, op->s));
+  spbte_wsum_args args = {NULL, FG, FG + stride, np == 2 ? FG + 2 * stride : NULL, np == 2 ? FG + 3 * stride : NULL,
                           op->d_Qh, ncell, np, 0};
   const scratch_set ss = own_scratch(op);
   spbte_tile best = {0, 0, 0, 0};

@@ -91,6 +91,7 @@ int spbte_launch_project_apply(const spbte_grid_dev *g, const double *masses, co
 int spbte_launch_wsum(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, hipStream_t s);
 int spbte_launch_wsum_scaled(int N, int form, const spbte_tile *tile, const spbte_wsum_args *a, const float *rowscale,
                              hipStream_t s);
+/* FP32 operands: n float2 values at out, then the n sums of their parts. */
 int spbte_launch_c64_to_c32(const void *in, void *out, size_t n, hipStream_t s);
 int spbte_launch_copy(const void *in, void *out, size_t bytes, hipStream_t s);
 /* n values of a fixed pseudo-random pattern in [-1, 1): float if single, else double */
