@@ -135,9 +135,11 @@ k_wsum_stream(const uint4 *__restrict__ Wp, const float *__restrict__ rowscale, 
   // and the two products share one MUL + FMA for the c (a + b) terms: 8
   // instructions per pair with w instead of 10.  The staged F entry holds
   // (c, d - c) in sF and c + d in sP; a + b comes from the sums stored after
-  // G's values.  (With one product the saving is 1 of 6, and the extra window
-  // registers cost more than it: ComputeQ keeps the plain form.)
-  constexpr bool GS = FORM == 3 && NP == 2;
+  // G's values.  One problem per lane only: with two, and with one product
+  // (a saving of 1 in 6), the extra window registers cost more than the
+  // instructions save (N=24, 512 problems, 3 rows x 3 waves x 2 problems:
+  // 39.4 ms against 30.1 ms in the plain form).
+  constexpr bool GS = FORM == 3 && NP == 2 && TB == 1;
   __shared__ float sP[GS ? NP * KB * P * N * LW : 1];
   constexpr bool SG = FORM != 0 || NP == 2 || TB >= 4;  // G by scalar loads (else staged in LDS)
   __shared__ __attribute__((aligned(16))) C sG[SG ? 1 : NP * KB * N2];  // [p][k][cy][cz]
