@@ -182,11 +182,13 @@ def main():
     # rate gives at the 2.4 GHz peak clock the published figures assume; per
     # (r, c) pair and problem the weighted sum issues 10 FMA/MUL instructions
     # for 18 flops (fused maxPreserve; 6 for 10 with one product), against 2
-    # flops per FMA; the FP32-product maxPreserve sum takes Gauss's form of the
-    # complex products, 8 instructions for the same 18 flops
+    # flops per FMA; the FP32-product maxPreserve sum at one problem per lane
+    # (streaming tiles with tb = 1) takes Gauss's form of the complex products,
+    # 8 instructions for the same 18 flops
     cyc = 2.0 * 64 * 1024 * ceil_ghz * 1e9 / (ceil_tf * 1e12)
     ceil_peak = 2.0 * 64 * 1024 * 2.4e9 / cyc / 1e12
-    instr = (8.0 if a.form == "fp32a" else 10.0) if wl.mp else 6.0
+    gauss = a.form == "fp32a" and wl.mp and tile[0] >= 1 and tile[3] == 1
+    instr = (8.0 if gauss else 10.0) if wl.mp else 6.0
     alg_ceiling = ceil_peak * flops_per_pair(wl.mp) / (2.0 * instr)
     log(f"issue ceiling ({'FP32' if a.form == 'fp32a' else 'FP64'} FMA chains): {ceil_tf:.1f} TFLOP/s at {ceil_ghz:.2f} GHz, "
         f"{cyc:.2f} cycles per wave FMA, {ceil_peak:.1f} TFLOP/s at 2.4 GHz; this instruction mix can reach "
