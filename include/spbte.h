@@ -39,9 +39,10 @@ enum {
  * FP16X2 streams W as two FP16 halves (hi + lo) with a power-of-two scale per
  * row, widened to FP64; FP32A streams W as float and also computes the products
  * in FP32: the batched (streaming) kernel sums 8 to 32 steps in FP32 and adds
- * each partial sum into an FP64 accumulator, the tiled kernel uses compensated
- * (TwoSum) FP32 sums.  Transforms, moments and the projection are FP64 in
- * every form. */
+ * each partial sum into an FP64 accumulator (its maxPreserve tiles with one
+ * problem per lane take each complex product in Gauss's three-multiplication
+ * form), the tiled kernel uses compensated (TwoSum) FP32 sums.  Transforms,
+ * moments and the projection are FP64 in every form. */
 enum { SPBTE_FORM_FP64 = 0, SPBTE_FORM_FP32 = 1, SPBTE_FORM_FP16X2 = 2, SPBTE_FORM_FP32A = 3 };
 
 /* flags of spbte_compute_q */
