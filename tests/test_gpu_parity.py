@@ -78,6 +78,11 @@ def test_qhat_fp64_random(spb, orc, N, ncell):
     Q = op.qhat(F, G, "fp64")
     for b in range(ncell):
         assert rel_l2(Q[b], orc.qhat(N, W, F[b], G[b])) < 1e-12
+    if N in (16, 24):
+        # FP32 products through the FP32 operand layout (values, then part sums)
+        Q32 = op.qhat(F, G, "fp32a")
+        for b in range(ncell):
+            assert rel_l2(Q32[b], Q[b]) < 1e-5, (b, rel_l2(Q32[b], Q[b]))
     if N in (9, 10, 12):
         # grids without reduced-precision tiles run FP64 only, and say so
         with pytest.raises(spb.SpbteError, match="reduced-precision"):
