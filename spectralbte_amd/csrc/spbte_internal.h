@@ -42,8 +42,11 @@ typedef struct {
                          sums of spbte_launch_moment_partials, 5 N per cell */
 } spbte_xform_job;
 
-/* Weighted-sum kernel arguments.  F/G arrays hold ncell * N^3 complex values
- * (double2 for form 0, float2 otherwise); Qh gets ncell * N^3 double2. */
+/* Weighted-sum kernel arguments.  F/G arrays hold ncell * N^3 complex values:
+ * double2 for the FP64-arithmetic forms; for the FP32-product form float2,
+ * each array followed by the ncell * N^3 sums of its values' parts (x + y),
+ * which its two-product streaming tiles read for G.  Qh gets ncell * N^3
+ * double2. */
 typedef struct {
   const void *W;      /* N^3 x N^3: double, float, or (hi, lo) half pairs */
   const void *F0, *G0, *F1, *G1;
