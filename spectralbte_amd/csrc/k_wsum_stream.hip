@@ -502,6 +502,8 @@ const SEntry kSTable[] = {
     // give one cell more waves
     S3(16, 4, 1, 1, 1), S4(16, 4, 1, 2, 1), S3(16, 4, 1, 1, 2), S4(16, 4, 1, 2, 2),
     S3(16, 2, 1, 1, 1), S3(16, 2, 1, 1, 2), S3(16, 1, 1, 1, 1), S3(16, 1, 1, 1, 2),
+    // one-row waves, two or four a workgroup (one cell: more waves, staging shared)
+    S3(16, 1, 2, 1, 1), S3(16, 1, 2, 1, 2), S3(16, 1, 4, 1, 1), S3(16, 1, 4, 1, 2),
 };
 #undef S4
 #undef S3

@@ -721,14 +721,16 @@ static long stream_wgs(int N, int ncell, const spbte_tile *t) {
  * as many as give 2048 workgroups, cutting x-planes into y-line ranges if
  * need be (one cell: 32 slices of half an x-plane, its FP64 table streamed in
  * 41.6 us against 48.3 with 16 whole planes and 99 with 8 slices of 4 rows;
- * 64 slices took 56.8 us, the F staging outweighing the extra waves). */
+ * 64 slices took 56.8 us, the F staging outweighing the extra waves; with the
+ * later kernel, two one-row waves a workgroup over the same 32 slices took
+ * 26.8 us against 29.5 with one). */
 static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile *t) {
   static const spbte_tile s32[] = {{4, 4, 4, 1}, {4, 4, 2, 1}, {4, 4, 1, 1}}, s32m[] = {{4, 4, 1, 1}},
                          s32m64[] = {{8, 2, 1, 1}};
   static const spbte_tile s24[] = {{4, 2, 2, 1}, {4, 2, 1, 1}}, s24m[] = {{4, 2, 1, 1}};
   static const spbte_tile s24ma[] = {{3, 3, 2, 1}, {3, 3, 1, 1}};
   static const spbte_tile s16[] = {{4, 1, 2, 1}, {4, 1, 1, 1}}, s16m[] = {{4, 1, 2, 1}, {4, 1, 1, 1}};
-  static const spbte_tile s16one[] = {{1, 1, 1, 1}};
+  static const spbte_tile s16one[] = {{1, 2, 1, 1}};
   const spbte_tile *c = NULL;
   int nc = 0;
   if (N == 32) {
@@ -740,7 +742,7 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
     else { c = nprod == 2 ? s24m : s24; nc = nprod == 2 ? 1 : 2; }
   }
   else if (N == 16) {
-    if (ncell <= 4 && spbte_stream_tile_supported(N, form, nprod, 1, 1, 1)) { c = s16one; nc = 1; }
+    if (ncell <= 4 && spbte_stream_tile_supported(N, form, nprod, 1, 2, 1)) { c = s16one; nc = 1; }
     else { c = nprod == 2 ? s16m : s16; nc = 2; }
   }
   else return 0;
@@ -750,7 +752,9 @@ static int default_stream_tile(int N, int form, int nprod, int ncell, spbte_tile
     *t = c[i];
     const long wgs = stream_wgs(N, ncell, t);
     int k = 1;
-    while (wgs * k < 2048 && spbte_csplit_valid(N, 2 * k) && (N <= 16 || 2 * k <= 8)) k *= 2;
+    /* N <= 16 counts waves, not workgroups (its one-row tiles run 2 waves a workgroup) */
+    while ((N <= 16 ? wgs * k * t->lines : wgs * k) < 2048 && spbte_csplit_valid(N, 2 * k) && (N <= 16 || 2 * k <= 8))
+      k *= 2;
     t->kind = k;
     return 1;
   }
@@ -1129,7 +1133,8 @@ static const spbte_tile k_cand[] = {
     /* one or a few cells at N=16: narrower rows, up to one x-plane per slice */
     {4, 1, 1, 8}, {4, 1, 1, 16}, {2, 1, 1, 8}, {2, 1, 1, 16}, {1, 1, 1, 8}, {1, 1, 1, 16}, {8, 2, 1, 16},
     /* and x-planes cut into y-line ranges */
-    {1, 1, 1, 32}, {1, 1, 1, 64}, {2, 1, 1, 32}, {2, 1, 1, 64}, {4, 1, 1, 64}, {4, 1, 2, 64}};
+    {1, 1, 1, 32}, {1, 1, 1, 64}, {2, 1, 1, 32}, {2, 1, 1, 64}, {4, 1, 1, 64}, {4, 1, 2, 64},
+    {1, 2, 1, 32}, {1, 2, 1, 64}, {1, 4, 1, 64}, {1, 4, 1, 128}};
 
 int spbte_autotune(spbte_op *op, int ncell, int form, int maxpreserve, int *kind, int *tr, int *lines, int *tb) {
   if (form < 0 || form >= NFORMS || ncell < 1) return fail(SPBTE_ERR_ARG, "bad arguments");

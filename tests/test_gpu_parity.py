@@ -103,7 +103,8 @@ def test_qhat_tiles_agree(spb, orc, N):
     op.set_weights(W)
     ref = np.stack([orc.qhat(N, W, F[b], G[b]).ravel() for b in range(ncell)])
     tiles = {16: [(4, 4, 1, 0), (4, 4, 2, 0), (4, 8, 2, 0), (4, 8, 4, 0), (4, 1, 1, 1), (4, 1, 2, 1),
-                  (1, 1, 1, 16), (1, 1, 1, 64), (2, 1, 1, 32), (4, 1, 2, 64)],   # x-planes cut into y-line ranges
+                  (1, 1, 1, 16), (1, 1, 1, 64), (2, 1, 1, 32), (4, 1, 2, 64),    # x-planes cut into y-line ranges
+                  (1, 2, 1, 32), (1, 4, 1, 64)],                                 # one-row waves, 2 and 4 a workgroup
              24: [(6, 8, 1, 0), (6, 8, 2, 0), (6, 8, 4, 0), (6, 3, 1, 1), (3, 3, 1, 1), (3, 3, 2, 1), (4, 2, 1, 1),
                   (4, 2, 2, 1), (8, 1, 1, 1)]}[N]
     for t in tiles:
