@@ -3,7 +3,9 @@
  * Replaces src/collisions.c.  Module state stays file-static and one grid per
  * process, as in the reference (src/collisions.c:11-30), but the operator runs
  * on the GPU: each call uploads f and g, evaluates the operator with the
- * batched path of spbte_core.c for one cell and downloads Q.  Weight tables
+ * batched path of spbte_core.c for one cell and downloads Q.  The copies go
+ * through a pinned host buffer as asynchronous copies on the operator's
+ * stream, so a call waits for the GPU once.  Weight tables
  * are cached in HBM keyed by the conv_weights pointer (the reference passes the
  * same pointer every step, exec/boltz.c:196,288), so an N=32 table (8.6 GB)
  * crosses PCIe once.
@@ -31,10 +33,18 @@ static int N = 0;
 static double L_v;
 static double *v, *eta;             /* borrowed */
 static double *d_buf;               /* device: f, g, Q (3 N^3) or one complex cube (2 N^3) */
+static double *h_buf;               /* pinned host staging of the same layout */
 static double g_mass = 1.0;
 
 static void die(const char *where) {
   fprintf(stderr, "spectralbte_amd: %s: %s\n", where, spbte_last_error());
+  fflush(stderr);
+  exit(1);
+}
+
+static void hip_ok(hipError_t e, const char *where) {
+  if (e == hipSuccess) return;
+  fprintf(stderr, "spectralbte_amd: %s: %s\n", where, hipGetErrorString(e));
   fflush(stderr);
   exit(1);
 }
@@ -80,6 +90,7 @@ void initialize_coll(int nodes, double length, double *vel, double *zeta) {
   eta = zeta;
   if (spbte_op_create(&op_plain, N, L_v, v, eta) != SPBTE_OK) die("initialize_coll");
   if (spbte_malloc((void **)&d_buf, sizeof(double) * 3 * (size_t)N * N * N) != SPBTE_OK) die("initialize_coll");
+  hip_ok(hipHostMalloc((void **)&h_buf, sizeof(double) * 3 * (size_t)N * N * N, hipHostMallocDefault), "initialize_coll");
 }
 
 void dealloc_coll(void) {
@@ -91,7 +102,9 @@ void dealloc_coll(void) {
   if (op_plain) spbte_op_destroy(op_plain);
   op_plain = NULL;
   if (d_buf) spbte_free(d_buf);
+  if (h_buf) hipHostFree(h_buf);
   d_buf = NULL;
+  h_buf = NULL;
   N = 0;
 }
 
@@ -142,16 +155,29 @@ static spbte_op *op_for(double **conv_weights) {
   return ops[slot];
 }
 
+/* Host -> device through the pinned buffer, ordered on stream s. */
+static void stage_in(double *d, double *h, const double *src, size_t bytes, hipStream_t s, const char *where) {
+  memcpy(h, src, bytes);
+  hip_ok(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s), where);
+}
+
+/* Device -> host through the pinned buffer after the work queued on s. */
+static void finish_out(double *dst, double *h, const double *d, size_t bytes, hipStream_t s, const char *where) {
+  hip_ok(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s), where);
+  hip_ok(hipStreamSynchronize(s), where);
+  memcpy(dst, h, bytes);
+}
+
 static void run_q(double *f, double *g, double *Q, double **conv_weights, int flags) {
   spbte_op *op = op_for(conv_weights);
-  const size_t n3 = (size_t)N * N * N;
+  hipStream_t s = spbte_op_stream(op);
+  const size_t n3 = (size_t)N * N * N, b = sizeof(double) * n3;
   double *df = d_buf, *dg = d_buf + n3, *dQ = d_buf + 2 * n3;
-  if (spbte_memcpy_htod(df, f, sizeof(double) * n3) != SPBTE_OK) die("ComputeQ");
-  if (g != f && spbte_memcpy_htod(dg, g, sizeof(double) * n3) != SPBTE_OK) die("ComputeQ");
+  stage_in(df, h_buf, f, b, s, "ComputeQ");
+  if (g != f) stage_in(dg, h_buf + n3, g, b, s, "ComputeQ");
   if (spbte_compute_q(op, df, g == f ? df : dg, dQ, 1, form_from_env(), flags | (g == f ? SPBTE_SELF : 0)) != SPBTE_OK)
     die("ComputeQ");
-  if (spbte_op_synchronize(op) != SPBTE_OK) die("ComputeQ");
-  if (spbte_memcpy_dtoh(Q, dQ, sizeof(double) * n3) != SPBTE_OK) die("ComputeQ");
+  finish_out(Q, h_buf + 2 * n3, dQ, b, s, "ComputeQ");
 }
 
 void ComputeQ(double *f, double *g, double *Q, double **conv_weights) { run_q(f, g, Q, conv_weights, 0); }
@@ -165,11 +191,11 @@ void fft3D(fftw_complex *in, fftw_complex *out, int invert) {
     fprintf(stderr, "spectralbte_amd: fft3D called before initialize_coll\n");
     exit(1);
   }
-  const size_t n3 = (size_t)N * N * N;
-  if (spbte_memcpy_htod(d_buf, in, 2 * sizeof(double) * n3) != SPBTE_OK) die("fft3D");
+  const size_t b = 2 * sizeof(double) * (size_t)N * N * N;
+  hipStream_t s = spbte_op_stream(op_plain);
+  stage_in(d_buf, h_buf, (const double *)in, b, s, "fft3D");
   if (spbte_transform(op_plain, d_buf, d_buf, 1, invert ? 1 : 0) != SPBTE_OK) die("fft3D");
-  if (spbte_op_synchronize(op_plain) != SPBTE_OK) die("fft3D");
-  if (spbte_memcpy_dtoh(out, d_buf, 2 * sizeof(double) * n3) != SPBTE_OK) die("fft3D");
+  finish_out((double *)out, h_buf, d_buf, b, s, "fft3D");
 }
 
 void ifft3D(fftw_complex *in, fftw_complex *out) { fft3D(in, out, 1); }

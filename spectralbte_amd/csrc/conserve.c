@@ -3,8 +3,10 @@
  * Replaces src/conserve.c.  C C^T (Ns+4 square) is built and LU-factored on
  * the host in FP64 once (src/conserve.c:268-317, :89-168), as the reference
  * does; conserveAllMoments uploads the Ns*Ns operator arrays, runs the moment
- * reduction, the solve and the correction on the GPU in FP64 (k_project in
- * k_transform.hip) and downloads the result.
+ * reduction, the solve and the correction on the GPU in FP64 (k_proj_part and
+ * k_proj_apply in k_transform.hip, one workgroup per x-plane) and downloads the
+ * result.  The arrays pass through a pinned host buffer, so each direction is
+ * one asynchronous copy and the call waits once.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -22,6 +24,8 @@ static int *pivot;
 static double masses[SPBTE_MAX_NS];
 static spbte_grid_dev grid; /* device copies of v and the trapezoid weights */
 static double *d_Q;
+static double *d_part;  /* (Ns+4) N per-plane partial sums */
+static double *h_Q;     /* pinned staging, Ns*Ns N^3 */
 static hipStream_t stream;
 
 static void die(const char *where, hipError_t e) {
@@ -54,8 +58,11 @@ static void setup(int nodes, double h_v, double *vel, int numspec, const double 
   if ((e = hipMalloc(&grid.d_w, sizeof(double) * N)) != hipSuccess) die("initialize_conservation", e);
   if ((e = hipMemcpy(grid.d_v, v, sizeof(double) * N, hipMemcpyHostToDevice)) != hipSuccess) die("initialize_conservation", e);
   if ((e = hipMemcpy(grid.d_w, w, sizeof(double) * N, hipMemcpyHostToDevice)) != hipSuccess) die("initialize_conservation", e);
-  if ((e = hipMalloc((void **)&d_Q, sizeof(double) * Ns * Ns * (size_t)N * N * N)) != hipSuccess)
+  const size_t qbytes = sizeof(double) * Ns * Ns * (size_t)N * N * N;
+  if ((e = hipMalloc((void **)&d_Q, qbytes)) != hipSuccess) die("initialize_conservation", e);
+  if ((e = hipMalloc((void **)&d_part, sizeof(double) * (Ns + 4) * (size_t)N)) != hipSuccess)
     die("initialize_conservation", e);
+  if ((e = hipHostMalloc((void **)&h_Q, qbytes, hipHostMallocDefault)) != hipSuccess) die("initialize_conservation", e);
   spbte_abi_set_mass(masses[0]);
   createCCtAndPivot();
 }
@@ -80,8 +87,12 @@ void dealloc_conservation(void) {
   hipFree(grid.d_v);
   hipFree(grid.d_w);
   hipFree(d_Q);
+  hipFree(d_part);
+  hipHostFree(h_Q);
   hipStreamDestroy(stream);
   d_Q = NULL;
+  d_part = NULL;
+  h_Q = NULL;
   N = 0;
 }
 
@@ -115,15 +126,15 @@ void conserveAllMoments(double **Q) {
     fprintf(stderr, "spectralbte_amd: conserveAllMoments called before initialize_conservation\n");
     exit(1);
   }
-  const size_t n3 = (size_t)N * N * N;
+  const size_t n3 = (size_t)N * N * N, na = (size_t)Ns * Ns;
   hipError_t e;
-  for (int a = 0; a < Ns * Ns; a++)
-    if ((e = hipMemcpyAsync(d_Q + a * n3, Q[a], sizeof(double) * n3, hipMemcpyHostToDevice, stream)) != hipSuccess)
-      die("conserveAllMoments", e);
-  int le = spbte_launch_project(&grid, Ns, masses, CCt, pivot, d_Q, 1, NULL, stream);
+  for (size_t a = 0; a < na; a++) memcpy(h_Q + a * n3, Q[a], sizeof(double) * n3);
+  if ((e = hipMemcpyAsync(d_Q, h_Q, sizeof(double) * na * n3, hipMemcpyHostToDevice, stream)) != hipSuccess)
+    die("conserveAllMoments", e);
+  int le = spbte_launch_project(&grid, Ns, masses, CCt, pivot, d_Q, 1, d_part, stream);
   if (le != 0) die("conserveAllMoments", (hipError_t)le);
-  for (int a = 0; a < Ns * Ns; a++)
-    if ((e = hipMemcpyAsync(Q[a], d_Q + a * n3, sizeof(double) * n3, hipMemcpyDeviceToHost, stream)) != hipSuccess)
-      die("conserveAllMoments", e);
+  if ((e = hipMemcpyAsync(h_Q, d_Q, sizeof(double) * na * n3, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+    die("conserveAllMoments", e);
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) die("conserveAllMoments", e);
+  for (size_t a = 0; a < na; a++) memcpy(Q[a], h_Q + a * n3, sizeof(double) * n3);
 }

@@ -494,6 +494,7 @@ __global__ void __launch_bounds__(XT) k_moments(int N, const double *__restrict_
 struct ProjArgs {
   double LU[(SPBTE_MAX_NS + 4) * (SPBTE_MAX_NS + 4)];
   double masses[SPBTE_MAX_NS];
+  double pm[SPBTE_MAX_NS];  // h_v^3 m per species
   int pivot[SPBTE_MAX_NS + 4];
 };
 
@@ -596,68 +597,90 @@ __global__ void __launch_bounds__(64) k_moments_fin(int N, int ncell, const doub
   moments_finish(s, hv, mass, kb, mom + (size_t)cell * SPBTE_MOM_STRIDE);
 }
 
-// One species: partial sums of p {1, v_x, v_y, v_z, |v|^2/2} Q over x-plane i,
-// p = w h_v^3 m (the rows of C, src/conserve.c:282-296).
+// Ns species, groups of Ns*Ns arrays at Q + grp * gstride, array n*Ns+m
+// (species m's operator against species n) pstride after the group's first.
+// Partial sums over x-plane i of p_m {1 (species m), v_x, v_y, v_z, |v|^2/2}
+// sum_n Q_{n*Ns+m}, p_m = w h_v^3 m_m: the rows of C (src/conserve.c:282-296),
+// NS + 4 values per (plane, group).
+template <int NS>
 __global__ void __launch_bounds__(XT) k_proj_part(int N, const double *__restrict__ v, const double *__restrict__ w,
-                                                  double pm, const double *__restrict__ Q, double *__restrict__ part) {
-  __shared__ double red[PS * (XT / 64)];
-  const int i = blockIdx.x, cell = blockIdx.y;
+                                                  ProjArgs pa, const double *__restrict__ Q, size_t gstride,
+                                                  size_t pstride, double *__restrict__ part) {
+  constexpr int NE = NS + 4;
+  __shared__ double red[NE * (XT / 64)];
+  const int i = blockIdx.x, grp = blockIdx.y;
   const int n2 = N * N;
-  const double *qp = Q + (size_t)cell * n2 * N + (size_t)i * n2;
-  const double vi = v[i], wi = w[i] * pm;
-  double s[PS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const double *qp = Q + (size_t)grp * gstride + (size_t)i * n2;
+  const double vi = v[i];
+  double wi[NS];
+  for (int m = 0; m < NS; ++m) wi[m] = w[i] * pa.pm[m];
+  double s[NE];
+  for (int q = 0; q < NE; ++q) s[q] = 0.0;
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int j = e / N, k = e - j * N;
-    const double p = wi * w[j] * w[k] * qp[e];
-    s[0] += p;
-    s[1] += p * vi;
-    s[2] += p * v[j];
-    s[3] += p * v[k];
-    s[4] += p * 0.5 * (vi * vi + v[j] * v[j] + v[k] * v[k]);
+    for (int m = 0; m < NS; ++m) {
+      double qs = qp[(size_t)m * pstride + e];
+      for (int n = 1; n < NS; ++n) qs += qp[(size_t)(n * NS + m) * pstride + e];
+      const double p = wi[m] * w[j] * w[k] * qs;
+      s[m] += p;
+      s[NS] += p * vi;
+      s[NS + 1] += p * v[j];
+      s[NS + 2] += p * v[k];
+      s[NS + 3] += p * 0.5 * (vi * vi + v[j] * v[j] + v[k] * v[k]);
+    }
   }
-  block_sum<PS>(s, red);
+  block_sum<NE>(s, red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < PS; ++q) part[((size_t)cell * N + i) * PS + q] = s[q];
+    for (int q = 0; q < NE; ++q) part[((size_t)grp * N + i) * NE + q] = s[q];
 }
 
-// Q <- Q - C^T (C C^T)^-1 C Q on x-plane i (src/conserve.c:207-264 with one
-// species); every workgroup of a cell combines the cell's N partials and
-// solves the 5 x 5 system itself.
+// Q <- Q - C^T (C C^T)^-1 C Q on x-plane i (src/conserve.c:207-264, the solve
+// of :173-202): every workgroup of a group combines the group's N partials and
+// solves the (NS+4)-square system itself; species m's correction is shared
+// equally by its NS operators.
+template <int NS>
 __global__ void __launch_bounds__(XT) k_proj_apply(int N, const double *__restrict__ v, const double *__restrict__ w,
-                                                   double pm, ProjArgs pa, double *__restrict__ Q,
-                                                   const double *__restrict__ part) {
-  __shared__ double lam[PS];
-  const int i = blockIdx.x, cell = blockIdx.y;
+                                                   ProjArgs pa, double *__restrict__ Q, size_t gstride,
+                                                   size_t pstride, const double *__restrict__ part) {
+  constexpr int NE = NS + 4;
+  __shared__ double lam[NE];
+  const int i = blockIdx.x, grp = blockIdx.y;
   const int n2 = N * N;
   if (threadIdx.x < 64) {  // wave 0 adds the partials (N <= 64), lane 0 solves
     const int l = threadIdx.x;
-    double b[PS];
-    for (int q = 0; q < PS; ++q) b[q] = l < N ? part[((size_t)cell * N + l) * PS + q] : 0.0;
-    for (int q = 0; q < PS; ++q)
+    double b[NE];
+    for (int q = 0; q < NE; ++q) b[q] = l < N ? part[((size_t)grp * N + l) * NE + q] : 0.0;
+    for (int q = 0; q < NE; ++q)
       for (int off = 32; off > 0; off >>= 1) b[q] += __shfl_xor(b[q], off, 64);
     if (l == 0) {
-      for (int k = 0; k < PS - 1; ++k) {
+      for (int k = 0; k < NE - 1; ++k) {
         const int p = pa.pivot[k];
         if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
-        for (int r = k + 1; r < PS; ++r) b[r] -= pa.LU[r * PS + k] * b[k];
+        for (int r = k + 1; r < NE; ++r) b[r] -= pa.LU[r * NE + k] * b[k];
       }
-      b[PS - 1] = b[PS - 1] / pa.LU[(PS - 1) * PS + PS - 1];
-      for (int r = PS - 2; r >= 0; --r) {
+      b[NE - 1] = b[NE - 1] / pa.LU[(NE - 1) * NE + NE - 1];
+      for (int r = NE - 2; r >= 0; --r) {
         double sum = 0.0;
-        for (int c = r + 1; c < PS; ++c) sum += pa.LU[r * PS + c] * b[c];
-        b[r] = 1.0 / pa.LU[r * PS + r] * (b[r] - sum);
+        for (int c = r + 1; c < NE; ++c) sum += pa.LU[r * NE + c] * b[c];
+        b[r] = 1.0 / pa.LU[r * NE + r] * (b[r] - sum);
       }
-      for (int q = 0; q < PS; ++q) lam[q] = b[q];
+      for (int q = 0; q < NE; ++q) lam[q] = b[q];
     }
   }
   __syncthreads();
-  double *qp = Q + (size_t)cell * n2 * N + (size_t)i * n2;
-  const double vi = v[i], wi = w[i] * pm;
+  double *qp = Q + (size_t)grp * gstride + (size_t)i * n2;
+  const double vi = v[i];
+  double wi[NS];
+  for (int m = 0; m < NS; ++m) wi[m] = w[i] * pa.pm[m];
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int j = e / N, k = e - j * N;
-    const double p = wi * w[j] * w[k];
     const double e2 = 0.5 * (vi * vi + v[j] * v[j] + v[k] * v[k]);
-    qp[e] -= p * lam[0] + p * vi * lam[1] + p * v[j] * lam[2] + p * v[k] * lam[3] + p * e2 * lam[4];
+    for (int m = 0; m < NS; ++m) {
+      const double p = wi[m] * w[j] * w[k] / NS;
+      const double corr = p * lam[m] + p * vi * lam[NS] + p * v[j] * lam[NS + 1] + p * v[k] * lam[NS + 2] +
+                          p * e2 * lam[NS + 3];
+      for (int n = 0; n < NS; ++n) qp[(size_t)(n * NS + m) * pstride + e] -= corr;
+    }
   }
 }
 
@@ -793,48 +816,66 @@ extern "C" int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *
   return (int)hipGetLastError();
 }
 
+static ProjArgs proj_args(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot) {
+  ProjArgs pa = {};
+  const int nel = Ns + 4;
+  for (int q = 0; q < nel * nel; ++q) pa.LU[q] = LU[q];
+  for (int q = 0; q < nel; ++q) pa.pivot[q] = pivot[q];
+  for (int q = 0; q < Ns; ++q) {
+    pa.masses[q] = masses[q];
+    pa.pm[q] = g->hv * g->hv * g->hv * masses[q];
+  }
+  return pa;
+}
+
+template <int NS>
+void launch_proj_split(const spbte_grid_dev *g, const ProjArgs &pa, double *Q, int ngroups, size_t gstride,
+                       size_t pstride, double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_proj_part<NS>, dim3(g->N, ngroups), dim3(XT), 0, s, g->N, (const double *)g->d_v,
+                     (const double *)g->d_w, pa, (const double *)Q, gstride, pstride, part);
+  hipLaunchKernelGGL(k_proj_apply<NS>, dim3(g->N, ngroups), dim3(XT), 0, s, g->N, (const double *)g->d_v,
+                     (const double *)g->d_w, pa, Q, gstride, pstride, (const double *)part);
+}
+
 extern "C" int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                                     double *Q, int ncell, double *part, hipStream_t s) {
   const size_t n3 = (size_t)g->N * g->N * g->N;
-  if (Ns != 1 || !part) return spbte_launch_project_strided(g, Ns, masses, LU, pivot, Q, ncell, (size_t)Ns * Ns * n3, n3, s);
-  ProjArgs pa = {};
-  for (int q = 0; q < PS * PS; ++q) pa.LU[q] = LU[q];
-  for (int q = 0; q < PS; ++q) pa.pivot[q] = pivot[q];
-  pa.masses[0] = masses[0];
-  const double pm = g->hv * g->hv * g->hv * masses[0];
-  hipLaunchKernelGGL(k_proj_part, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v, (const double *)g->d_w,
-                     pm, (const double *)Q, part);
-  hipLaunchKernelGGL(k_proj_apply, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
-                     (const double *)g->d_w, pm, pa, Q, (const double *)part);
-  return (int)hipGetLastError();
+  return spbte_launch_project_strided(g, Ns, masses, LU, pivot, Q, ncell, (size_t)Ns * Ns * n3, n3, part, s);
 }
 
 extern "C" int spbte_launch_project_apply(const spbte_grid_dev *g, const double *masses, const double *LU,
                                           const int *pivot, double *Q, int ncell, const double *part, hipStream_t s) {
-  ProjArgs pa = {};
-  for (int q = 0; q < PS * PS; ++q) pa.LU[q] = LU[q];
-  for (int q = 0; q < PS; ++q) pa.pivot[q] = pivot[q];
-  pa.masses[0] = masses[0];
-  const double pm = g->hv * g->hv * g->hv * masses[0];
-  hipLaunchKernelGGL(k_proj_apply, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
-                     (const double *)g->d_w, pm, pa, Q, part);
+  const size_t n3 = (size_t)g->N * g->N * g->N;
+  const ProjArgs pa = proj_args(g, 1, masses, LU, pivot);
+  hipLaunchKernelGGL(k_proj_apply<1>, dim3(g->N, ncell), dim3(XT), 0, s, g->N, (const double *)g->d_v,
+                     (const double *)g->d_w, pa, Q, n3, n3, part);
   return (int)hipGetLastError();
 }
 
 // Groups at Q + grp * gstride, the Ns^2 pairs of a group pstride apart
 // (group-major: gstride = Ns^2 N^3, pstride = N^3; pair-major over ncell
-// cells: gstride = N^3, pstride = ncell N^3).
+// cells: gstride = N^3, pstride = ncell N^3).  With a partial-sum buffer
+// ((Ns+4) N doubles per group) every (x-plane, group) is a workgroup of its
+// own; without one, each group is one workgroup.
 extern "C" int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU,
                                             const int *pivot, double *Q, int ngroups, size_t gstride, size_t pstride,
-                                            hipStream_t s) {
+                                            double *part, hipStream_t s) {
   if (Ns < 1 || Ns > SPBTE_MAX_NS) return (int)hipErrorInvalidValue;
-  const int ncell = ngroups;
-  ProjArgs pa = {};
-  const int nel = Ns + 4;
-  for (int q = 0; q < nel * nel; ++q) pa.LU[q] = LU[q];
-  for (int q = 0; q < nel; ++q) pa.pivot[q] = pivot[q];
-  for (int q = 0; q < Ns; ++q) pa.masses[q] = masses[q];
-  hipLaunchKernelGGL(k_project, dim3(ncell), dim3(XT), 0, s, g->N, Ns, (const double *)g->d_v, (const double *)g->d_w, g->hv,
+  const ProjArgs pa = proj_args(g, Ns, masses, LU, pivot);
+  if (part && ngroups <= 65535) {
+    switch (Ns) {
+      case 1: launch_proj_split<1>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 2: launch_proj_split<2>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 3: launch_proj_split<3>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 4: launch_proj_split<4>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 5: launch_proj_split<5>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 6: launch_proj_split<6>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      case 7: launch_proj_split<7>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+      default: launch_proj_split<8>(g, pa, Q, ngroups, gstride, pstride, part, s); break;
+    }
+    return (int)hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_project, dim3(ngroups), dim3(XT), 0, s, g->N, Ns, (const double *)g->d_v, (const double *)g->d_w, g->hv,
                      pa, Q, gstride, pstride);
   return (int)hipGetLastError();
 }

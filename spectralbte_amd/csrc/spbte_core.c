@@ -1003,13 +1003,18 @@ int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, 
   return compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags, NULL, NULL, &ss);
 }
 
+/* The operator's partial-sum scratch when it holds the split projection's
+ * (Ns+4) N doubles per group, else NULL (one workgroup per group). */
+static double *proj_part(const spbte_op *op, int Ns, int ngroups) {
+  return op->d_part && (size_t)(Ns + 4) * ngroups <= (size_t)10 * op->cap ? op->d_part : NULL;
+}
+
 int spbte_conserve(spbte_op *op, double *d_Q, int ncell) {
   if (!op || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ncell > 0 && !d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
   if (op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
   if (ncell <= 0) return SPBTE_OK;
-  LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ncell <= op->cap ? op->d_part : NULL,
-                            op->s));
+  LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, proj_part(op, 1, ncell), op->s));
   return SPBTE_OK;
 }
 
@@ -1017,7 +1022,7 @@ int spbte_conserve_pairs(spbte_op *op, double *d_Q, int ncell) {
   if (!op || !d_Q || ncell < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ncell == 0) return SPBTE_OK;
   LCHK(spbte_launch_project_strided(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ncell, op->n3,
-                                    (size_t)ncell * op->n3, op->s));
+                                    (size_t)ncell * op->n3, proj_part(op, op->Ns, ncell), op->s));
   return SPBTE_OK;
 }
 
@@ -1033,7 +1038,8 @@ int spbte_conserve_species(spbte_op *op, double *d_Q, int ngroups) {
   if (!op || ngroups < 0) return fail(SPBTE_ERR_ARG, "bad arguments");
   if (ngroups == 0) return SPBTE_OK;
   if (!d_Q) return fail(SPBTE_ERR_ARG, "null buffer");
-  LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, NULL, op->s));
+  LCHK(spbte_launch_project(&op->g, op->Ns, op->masses, op->LU, op->pivot, d_Q, ngroups, proj_part(op, op->Ns, ngroups),
+                            op->s));
   return SPBTE_OK;
 }
 

@@ -108,10 +108,13 @@ int spbte_launch_wsum_stream(int N, int form, int tr, int waves, int tb, const s
                              const float *rowscale, int csplit, void *Qp, hipStream_t s);
 int spbte_launch_permute_weights(const double *W64, void *dst, const float *rowscale, int form, int TR, size_t n3,
                                  hipStream_t s);
+/* Projection of ncell groups of Ns*Ns arrays (group-major).  part: (Ns+4) N
+ * doubles per group of scratch for the split (x-plane, group) kernels, or
+ * NULL for one workgroup per group. */
 int spbte_launch_project(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
                          double *Q, int ncell, double *part, hipStream_t s);
 int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, const double *masses, const double *LU, const int *pivot,
-                                 double *Q, int ngroups, size_t gstride, size_t pstride, hipStream_t s);
+                                 double *Q, int ngroups, size_t gstride, size_t pstride, double *part, hipStream_t s);
 int spbte_launch_convert_weights(const double *W64, void *dst, int form, size_t n_rows, size_t n_cols, hipStream_t s);
 int spbte_launch_weights_iso(const spbte_grid_dev *g, const double *eta, double lam, double dfac, double mu,
                              double *W, hipStream_t s);
