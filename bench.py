@@ -66,6 +66,8 @@ def parse():
     p.add_argument("--no-extra", action="store_true", help="skip every row beside the headline")
     p.add_argument("--no-runs", action="store_true", help="skip the whole-step runs (shock, mixed ensemble)")
     p.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
+    p.add_argument("--rows", default=None,
+                   help="comma-separated secondary rows to run, e.g. c1_bkw16 (default: every row the flags above keep)")
     return p.parse_args()
 
 
@@ -227,17 +229,22 @@ def main():
                       "speedup_vs_plain_fp64": round(value / plain, 2)})
 
     # the secondary measurements report an error instead of costing the headline line
+    rows = None if a.rows is None else set(a.rows.split(","))
+    def row(key, label, fn):
+        if rows is None or key in rows:
+            extra[key] = guarded(label, fn, spb)
+
     if not a.no_extra:
-        extra["c1_bkw16"] = guarded("BKW16 run", c1_bkw16, spb)
-        extra["c3_shock_forms"] = guarded("shock collision step per form", c3_shock_forms, spb)
+        row("c1_bkw16", "BKW16 run", c1_bkw16)
+        row("c3_shock_forms", "shock collision step per form", c3_shock_forms)
     if not a.no_extra and not a.no_runs:
-        extra["shock_run"] = guarded("shock run", shock_run, spb)
-        extra["ensemble_mixed_run"] = guarded("mixed ensemble run", mixed_ensemble_run, spb)
-        extra["landau_table"] = guarded("Landau table", landau_table, spb)
-        extra["glance_table"] = guarded("glancing-collision table", glance_table, spb)
-        extra["heat_glance_run"] = guarded("heat-transfer run", heat_glance_run, spb)
+        row("shock_run", "shock run", shock_run)
+        row("ensemble_mixed_run", "mixed ensemble run", mixed_ensemble_run)
+        row("landau_table", "Landau table", landau_table)
+        row("glance_table", "glancing-collision table", glance_table)
+        row("heat_glance_run", "heat-transfer run", heat_glance_run)
     if not a.no_extra and not a.no_surrogate:
-        extra["surrogate"] = guarded("surrogate", surrogate_run, spb)
+        row("surrogate", "surrogate", surrogate_run)
 
     cpu = None
     if not a.no_cpu:

@@ -312,21 +312,58 @@ __device__ void load_tabs(FftTabs &t, const double2 *__restrict__ g) {
 // of (cell, job) times p(i) p(j) p(k), then z and y FFTs, to scratch.
 template <int N>
 __global__ void __launch_bounds__(XT) k_fft_a_real(XformJobs jobs, int ncell, const double2 *__restrict__ tabs,
-                                                   const double *__restrict__ v, double2 *__restrict__ scratch, double dsgn,
-                                                   double hv, double mass, double kb) {
+                                                   const double *__restrict__ v, const double *__restrict__ w,
+                                                   double2 *__restrict__ scratch, double dsgn, double hv, double mass,
+                                                   double kb) {
   __shared__ double2 A[N * N], B[N * N];
   __shared__ FftTabs t;
   __shared__ double smom[SPBTE_MOM_STRIDE];
+  __shared__ double red[5 * (XT / 64)];
+  __shared__ double sv[N], sw[N];
   constexpr int n2 = N * N;
   constexpr size_t n3 = (size_t)n2 * N;
   const int i = blockIdx.x, cell = blockIdx.y, jb = blockIdx.z;
   const spbte_xform_job &job = jobs.j[jb];
   load_tabs<N>(t, tabs);
-  // the Maxwellian's moments: from the cell's per-plane partial sums, or given
+  // the Maxwellian's moments: from the cell's per-plane partial sums, from
+  // the whole cell of msrc (small grids: every workgroup of the job sums it),
+  // or given
+  const bool inl = N <= 16 && job.mode != 0 && !job.part && !job.mom;  // spbte_fft_moments_inline
   if (job.part && job.mode != 0 && threadIdx.x < 64)
     moments_wave(N, job.part + (size_t)cell * N * 5, hv, mass, kb, smom);
+  if constexpr (N <= 16) if (inl) {  // all of the cell's loads issued at once, v and w from LDS
+    constexpr int R = (int)((n3 + XT - 1) / XT);
+    const double *ms = job.msrc + cell * n3;
+    double fv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = threadIdx.x + r * XT;
+      fv[r] = e < (int)n3 ? ms[e] : 0.0;
+    }
+    if (threadIdx.x < N) {
+      sv[threadIdx.x] = v[threadIdx.x];
+      sw[threadIdx.x] = w[threadIdx.x];
+    }
+    __syncthreads();
+    double s5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = threadIdx.x + r * XT;
+      if (e < (int)n3) {
+        const int a = e / n2, b = (e / N) % N, c = e % N;
+        const double ww = sw[a] * sw[b] * sw[c] * fv[r];
+        s5[0] += ww;
+        s5[1] += sv[a] * ww;
+        s5[2] += sv[b] * ww;
+        s5[3] += sv[c] * ww;
+        s5[4] += (sv[a] * sv[a] + sv[b] * sv[b] + sv[c] * sv[c]) * ww;
+      }
+    }
+    block_sum<5>(s5, red);
+    if (threadIdx.x == 0) moments_finish(s5, hv, mass, kb, smom);
+  }
   __syncthreads();
-  const double *mom = job.part ? smom : (job.mom ? job.mom + (size_t)cell * SPBTE_MOM_STRIDE : nullptr);
+  const double *mom = job.part || inl ? smom : (job.mom ? job.mom + (size_t)cell * SPBTE_MOM_STRIDE : nullptr);
   const double2 pi_ = t.p[i];
   for (int e = threadIdx.x; e < n2; e += XT) {
     const int j = e / N, k = e - j * N;
@@ -597,6 +634,23 @@ __global__ void __launch_bounds__(64) k_moments_fin(int N, int ncell, const doub
   moments_finish(s, hv, mass, kb, mom + (size_t)cell * SPBTE_MOM_STRIDE);
 }
 
+// b <- (C C^T)^-1 b from the host's LU factors with their row interchanges
+// (forward elimination, then back substitution; src/conserve.c:173-202).
+template <int NE>
+__device__ __forceinline__ void proj_solve(double (&b)[NE], const ProjArgs &pa) {
+  for (int k = 0; k < NE - 1; ++k) {
+    const int p = pa.pivot[k];
+    if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
+    for (int r = k + 1; r < NE; ++r) b[r] -= pa.LU[r * NE + k] * b[k];
+  }
+  b[NE - 1] = b[NE - 1] / pa.LU[(NE - 1) * NE + NE - 1];
+  for (int r = NE - 2; r >= 0; --r) {
+    double sum = 0.0;
+    for (int c = r + 1; c < NE; ++c) sum += pa.LU[r * NE + c] * b[c];
+    b[r] = 1.0 / pa.LU[r * NE + r] * (b[r] - sum);
+  }
+}
+
 // Ns species, groups of Ns*Ns arrays at Q + grp * gstride, array n*Ns+m
 // (species m's operator against species n) pstride after the group's first.
 // Partial sums over x-plane i of p_m {1 (species m), v_x, v_y, v_z, |v|^2/2}
@@ -653,17 +707,7 @@ __global__ void __launch_bounds__(XT) k_proj_apply(int N, const double *__restri
     for (int q = 0; q < NE; ++q)
       for (int off = 32; off > 0; off >>= 1) b[q] += __shfl_xor(b[q], off, 64);
     if (l == 0) {
-      for (int k = 0; k < NE - 1; ++k) {
-        const int p = pa.pivot[k];
-        if (p != k) { const double t = b[p]; b[p] = b[k]; b[k] = t; }
-        for (int r = k + 1; r < NE; ++r) b[r] -= pa.LU[r * NE + k] * b[k];
-      }
-      b[NE - 1] = b[NE - 1] / pa.LU[(NE - 1) * NE + NE - 1];
-      for (int r = NE - 2; r >= 0; --r) {
-        double sum = 0.0;
-        for (int c = r + 1; c < NE; ++c) sum += pa.LU[r * NE + c] * b[c];
-        b[r] = 1.0 / pa.LU[r * NE + r] * (b[r] - sum);
-      }
+      proj_solve<NE>(b, pa);
       for (int q = 0; q < NE; ++q) lam[q] = b[q];
     }
   }
@@ -684,6 +728,120 @@ __global__ void __launch_bounds__(XT) k_proj_apply(int N, const double *__restri
   }
 }
 
+// ---- the inverse transform's last pass over whole cubes for small grids
+// (N^2 <= XT): one workgroup per cube, thread t owning the x-line t in
+// registers, so the real part can be projected in the same launch (the
+// projection needs the whole cube's sums).
+
+// In-register length-N FFT (the Stockham stages of fft_axis, unrolled):
+// x_t <- sum_r x_r exp(dsgn 2 pi i r t / N), tw[m] = exp(dsgn 2 pi i m / N).
+template <int N, int ST, int SPAN>
+__device__ __forceinline__ void dft_reg_stage(double2 (&a)[N], double2 (&b)[N], const double2 *tw, double dsgn) {
+  if constexpr (ST < fft_nstages(N)) {
+    constexpr int R = fft_radix(N, ST);
+    constexpr int NB = N / R;
+    constexpr int step = N / (SPAN * R);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int k = j % SPAN;
+      double2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = a[j + r * NB];
+#pragma unroll
+      for (int r = 1; r < R; ++r)
+        if (k * r * step != 0) v[r] = cmul(v[r], tw[k * r * step]);
+      butterfly<R>(v, dsgn);
+      const int o = (j / SPAN) * SPAN * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) b[o + r * SPAN] = v[r];
+    }
+    dft_reg_stage<N, ST + 1, SPAN * R>(b, a, tw, dsgn);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void dft_reg(double2 (&x)[N], const double2 *tw, double dsgn) {
+  double2 y[N];
+  dft_reg_stage<N, 0, 1>(x, y, tw, dsgn);
+  if constexpr (fft_nstages(N) % 2 == 1) {
+#pragma unroll
+    for (int m = 0; m < N; ++m) x[m] = y[m];
+  }
+}
+
+// Last pass of the inverse transform of whole cubes: pass A (k_fft_a_cplx)
+// has done z and y; workgroup `cell` does x, keeps the real part and, with
+// PROJ, projects it (src/conserve.c:207-264 with one species: the moments'
+// sums, the 5 x 5 solve and the correction, all in this workgroup) before the
+// one store of Q.  With ep.out, also out = a x + b y + c Q over the same
+// cells (a time step's update; x and out may be the same array).
+template <int N, bool PROJ>
+__global__ void __launch_bounds__(XT) k_fft_cube_inv_x(int ncell, const double2 *__restrict__ tabs, double pre,
+                                                       const double2 *__restrict__ scratch, double *__restrict__ q,
+                                                       const double *__restrict__ v, const double *__restrict__ w,
+                                                       double pm, ProjArgs pa, spbte_epilogue ep) {
+  static_assert(N * N <= XT, "one line per thread");
+  constexpr int n2 = N * N;
+  constexpr size_t n3 = (size_t)n2 * N;
+  constexpr double dsgn = 1.0;
+  __shared__ FftTabs t;
+  __shared__ double red[5 * (XT / 64)];
+  __shared__ double lam[5];
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const bool own = tid < n2;
+  const int b = own ? tid / N : 0, c = own ? tid - b * N : 0;
+  load_tabs<N>(t, tabs);
+  double2 x[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) x[a] = own ? scratch[cell * n3 + (size_t)a * n2 + tid] : make_double2(0.0, 0.0);
+  __syncthreads();
+  double qv[N];
+  dft_reg<N>(x, t.tw, dsgn);
+#pragma unroll
+  for (int a = 0; a < N; ++a) qv[a] = pre * cmul(cmul(cmul(t.q[a], t.q[b]), t.q[c]), x[a]).x;
+  if constexpr (PROJ) {
+    double ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (own) {
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        const double p = w[a] * w[b] * w[c] * pm * qv[a];
+        ps[0] += p;
+        ps[1] += p * v[a];
+        ps[2] += p * v[b];
+        ps[3] += p * v[c];
+        ps[4] += p * 0.5 * (v[a] * v[a] + v[b] * v[b] + v[c] * v[c]);
+      }
+    }
+    block_sum<5>(ps, red);
+    if (tid == 0) {
+      proj_solve<5>(ps, pa);
+      for (int m = 0; m < 5; ++m) lam[m] = ps[m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      const double p = w[a] * pm * w[b] * w[c];
+      const double e2 = 0.5 * (v[a] * v[a] + v[b] * v[b] + v[c] * v[c]);
+      qv[a] -= p * lam[0] + p * v[a] * lam[1] + p * v[b] * lam[2] + p * v[c] * lam[3] + p * e2 * lam[4];
+    }
+  }
+  if (own) {
+#pragma unroll
+    for (int a = 0; a < N; ++a) q[cell * n3 + (size_t)a * n2 + tid] = qv[a];
+    if (ep.out) {  // every load before the first store: out may alias x
+      double r[N];
+#pragma unroll
+      for (int a = 0; a < N; ++a) r[a] = ep.a * ep.x[cell * n3 + (size_t)a * n2 + tid];
+      if (ep.y) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) r[a] += ep.b * ep.y[cell * n3 + (size_t)a * n2 + tid];
+      }
+#pragma unroll
+      for (int a = 0; a < N; ++a) ep.out[cell * n3 + (size_t)a * n2 + tid] = r[a] + ep.c * qv[a];
+    }
+  }
+}
+
 // FFT-capable grids (N = 2^a 3^b with both LDS buffers of a plane in 64 KiB).
 constexpr int kFftN[] = {6, 8, 9, 12, 16, 18, 24, 32, 36, 48};
 
@@ -692,7 +850,7 @@ void launch_fft_fwd(const spbte_grid_dev *g, const XformJobs &J, int njobs, int 
                     hipStream_t s) {
   const double2 *tf = (const double2 *)g->d_fft;  // forward p, q, tw
   hipLaunchKernelGGL(k_fft_a_real<N>, dim3(N, ncell, njobs), dim3(XT), 0, s, J, ncell, tf, (const double *)g->d_v,
-                     (double2 *)scratch, -1.0, g->hv, g->mass, g->kb);
+                     (const double *)g->d_w, (double2 *)scratch, -1.0, g->hv, g->mass, g->kb);
   if (out_fp32)
     hipLaunchKernelGGL((k_fft_b<N, 1>), dim3(N, ncell, njobs), dim3(XT), 0, s, ncell, tf, g->pre_fwd,
                        (const double2 *)scratch, J, nullptr, -1.0, nullptr, nullptr, 0.0, nullptr);
@@ -718,6 +876,39 @@ void launch_fft_cplx(const spbte_grid_dev *g, const void *in, int nsl, void *out
   else
     hipLaunchKernelGGL((k_fft_b<N, 0>), dim3(N, ncell, 1), dim3(XT), 0, s, ncell, t, pre, (const double2 *)scratch, J,
                        out, dsgn, nullptr, nullptr, 0.0, nullptr);
+}
+
+// Grids with the whole-cube last pass (N^2 <= XT), used when a launch has at
+// most kCubeMaxWg cubes: more cubes fill the GPU with the plane passes as well.
+constexpr int kCubeN[] = {6, 8, 9, 12, 16};
+constexpr int kCubeMaxWg = 512;
+
+template <typename F, int I = 0>
+bool with_cube_n(int N, F &&fn) {
+  if constexpr (I < (int)(sizeof kCubeN / sizeof kCubeN[0])) {
+    if (N == kCubeN[I]) {
+      fn(std::integral_constant<int, kCubeN[I]>{});
+      return true;
+    }
+    return with_cube_n<F, I + 1>(N, static_cast<F &&>(fn));
+  } else {
+    return false;
+  }
+}
+
+template <int N>
+void launch_cube_inv(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell, void *scratch,
+                     const ProjArgs *pa, const spbte_epilogue &ep, hipStream_t s) {
+  const double2 *t = (const double2 *)g->d_fft + 3 * N;  // inverse p, q, tw
+  hipLaunchKernelGGL(k_fft_a_cplx<N>, dim3(N, ncell, 1), dim3(XT), 0, s, (const double2 *)Qh, nsl, ncell, t,
+                     (double2 *)scratch, 1.0);
+  const double *v = (const double *)g->d_v, *w = (const double *)g->d_w;
+  if (pa)
+    hipLaunchKernelGGL((k_fft_cube_inv_x<N, true>), dim3(ncell), dim3(XT), 0, s, ncell, t, g->pre_inv,
+                       (const double2 *)scratch, q, v, w, pa->pm[0], *pa, ep);
+  else
+    hipLaunchKernelGGL((k_fft_cube_inv_x<N, false>), dim3(ncell), dim3(XT), 0, s, ncell, t, g->pre_inv,
+                       (const double2 *)scratch, q, v, w, 0.0, ProjArgs{}, ep);
 }
 
 // Runs fn<N>() for the grid's N if it has FFT kernels; false otherwise.
@@ -762,6 +953,9 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
   if (njobs < 1 || njobs > 4) return (int)hipErrorInvalidValue;
   XformJobs J = {};
   for (int q = 0; q < njobs; ++q) J.j[q] = jobs[q];
+  for (int q = 0; q < njobs; ++q)
+    if (jobs[q].mode != 0 && !jobs[q].mom && !jobs[q].part && (!jobs[q].msrc || !spbte_fft_moments_inline(g->N)))
+      return (int)hipErrorInvalidValue;
   if (g->d_fft && with_fft_n(g->N, [&](auto n) {
         launch_fft_fwd<decltype(n)::value>(g, J, njobs, ncell, out_fp32, scratch, s);
       }))
@@ -877,6 +1071,26 @@ extern "C" int spbte_launch_project_strided(const spbte_grid_dev *g, int Ns, con
   }
   hipLaunchKernelGGL(k_project, dim3(ngroups), dim3(XT), 0, s, g->N, Ns, (const double *)g->d_v, (const double *)g->d_w, g->hv,
                      pa, Q, gstride, pstride);
+  return (int)hipGetLastError();
+}
+
+// Grids small enough for every forward-transform workgroup of a Maxwellian job
+// to sum its cell's moments itself (msrc jobs), which saves the partial-sum pass.
+extern "C" int spbte_fft_moments_inline(int N) { return N <= 16 && spbte_fft_supported(N); }
+
+extern "C" int spbte_fft_cube(int N, int ncubes) {
+  return ncubes >= 1 && ncubes <= kCubeMaxWg && with_cube_n(N, [](auto) {}) ? 1 : 0;
+}
+
+extern "C" int spbte_launch_xform_inv_real_cube(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell,
+                                                void *scratch, const double *masses, const double *LU, const int *pivot,
+                                                const spbte_epilogue *epi, hipStream_t s) {
+  if (nsl < 1 || !g->d_fft || !spbte_fft_cube(g->N, ncell)) return (int)hipErrorInvalidValue;
+  const ProjArgs pa = masses ? proj_args(g, 1, masses, LU, pivot) : ProjArgs{};
+  const spbte_epilogue ep = epi ? *epi : spbte_epilogue{};
+  with_cube_n(g->N, [&](auto n) {
+    launch_cube_inv<decltype(n)::value>(g, Qh, nsl, q, ncell, scratch, masses ? &pa : nullptr, ep, s);
+  });
   return (int)hipGetLastError();
 }
 

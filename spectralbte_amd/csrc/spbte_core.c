@@ -867,7 +867,7 @@ static int run_wsum(spbte_op *op, int form, int nprod, const void *F0, const voi
 /* One chunk of spbte_compute_q.  ev_a/ev_b (optional) bracket the weighted-sum
  * launch so its duration can be timed on the operator's stream. */
 static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, double *d_Q, int ncell, int form, int flags,
-                         hipEvent_t ev_a, hipEvent_t ev_b, const scratch_set *ss) {
+                         hipEvent_t ev_a, hipEvent_t ev_b, const scratch_set *ss, const spbte_epilogue *epi) {
   const int self = (flags & SPBTE_SELF) != 0 || d_g == d_f;
   const int mp = (flags & SPBTE_MAXPRESERVE) != 0;
   const size_t n = (size_t)ncell * op->n3;
@@ -883,34 +883,42 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
    * from per-plane partial sums (f's, then g's, in ss->part), and the inverse
    * transform writes the projection's partial sums (over f's, dead by then) */
   const int fused = op->g.d_fft != NULL && spbte_fft_supported(op->N);
-  const double *part_f = fused ? ss->part : NULL;
-  const double *part_g = fused ? ss->part + (size_t)5 * op->N * ncell : NULL;
+  /* small grids: the forward transform's workgroups take the Maxwellians'
+   * moments from the cells themselves, and the inverse transform's last pass
+   * runs over whole cubes and projects in the same launch (few cells) */
+  const int mom_inl = fused && spbte_fft_moments_inline(op->N);
+  const int cube_i = fused && spbte_fft_cube(op->N, ncell);
+  const double *part_f = fused && !mom_inl ? ss->part : NULL;
+  const double *part_g = fused && !mom_inl ? ss->part + (size_t)5 * op->N * ncell : NULL;
+  const double *msrc_f = mom_inl ? d_f : NULL, *msrc_g = mom_inl ? d_g : NULL;
   if (!mp) {
-    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL};
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL, NULL};
     F0 = B[0];
     if (self) G0 = B[0];
-    else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1], NULL}; G0 = B[1]; }
+    else { jobs[nj++] = (spbte_xform_job){d_g, NULL, 0, B[1], NULL, NULL}; G0 = B[1]; }
   } else {
     /* Q = sum W [ T(g - M_f)[c] T(f)[s] + T(M_g)[c] T(f - M_f)[s] ] */
     if (fused) {
-      LCHK(spbte_launch_moment_partials(&op->g, d_f, ncell, (double *)part_f, op->s));
-      if (!self) LCHK(spbte_launch_moment_partials(&op->g, d_g, ncell, (double *)part_g, op->s));
+      if (!mom_inl) {
+        LCHK(spbte_launch_moment_partials(&op->g, d_f, ncell, (double *)part_f, op->s));
+        if (!self) LCHK(spbte_launch_moment_partials(&op->g, d_g, ncell, (double *)part_g, op->s));
+      }
       mom_f = mom_g = NULL;
     } else {
       LCHK(spbte_launch_moments(&op->g, d_f, mom_f, ncell, ss->part, op->s));
       if (!self) LCHK(spbte_launch_moments(&op->g, d_g, mom_g, ncell, ss->part, op->s));
     }
-    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL};         /* F0 = T(f)        */
-    jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1], part_f};      /* F1 = T(f - M_f)  */
+    jobs[nj++] = (spbte_xform_job){d_f, NULL, 0, B[0], NULL, NULL};           /* F0 = T(f)        */
+    jobs[nj++] = (spbte_xform_job){d_f, mom_f, 1, B[1], part_f, msrc_f};      /* F1 = T(f - M_f)  */
     F0 = B[0];
     F1 = B[1];
     if (self) {
       G0 = B[1];
-      jobs[nj++] = (spbte_xform_job){d_f, mom_f, 2, B[2], part_f};    /* G1 = T(M_f)      */
+      jobs[nj++] = (spbte_xform_job){d_f, mom_f, 2, B[2], part_f, msrc_f};    /* G1 = T(M_f)      */
       G1 = B[2];
     } else {
-      jobs[nj++] = (spbte_xform_job){d_g, mom_f, 1, B[2], part_f};    /* G0 = T(g - M_f)  */
-      jobs[nj++] = (spbte_xform_job){d_g, mom_g, 2, B[3], part_g};    /* G1 = T(M_g)      */
+      jobs[nj++] = (spbte_xform_job){d_g, mom_f, 1, B[2], part_f, msrc_f};    /* G0 = T(g - M_f)  */
+      jobs[nj++] = (spbte_xform_job){d_g, mom_g, 2, B[3], part_g, msrc_g};    /* G1 = T(M_g)      */
       G0 = B[2];
       G1 = B[3];
     }
@@ -925,6 +933,12 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
   if (rc) return rc;
   const int cons = (flags & SPBTE_CONSERVE) != 0;
   if (cons && op->Ns != 1) return fail(SPBTE_ERR_UNSUPPORTED, "per-cell projection needs one species (Ns=%d)", op->Ns);
+  if (cube_i) {
+    LCHK(spbte_launch_xform_inv_real_cube(&op->g, qsrc, nsl, d_Q, ncell, ss->xs, cons ? op->masses : NULL, op->LU,
+                                          op->pivot, epi, op->s));
+    return SPBTE_OK;
+  }
+  if (epi) return fail(SPBTE_ERR_UNSUPPORTED, "no fused update on this grid and batch");
   LCHK(spbte_launch_xform_inv_real(&op->g, qsrc, nsl, d_Q, ncell, ss->xs, cons && fused ? ss->part : NULL, op->s));
   if (cons && fused) LCHK(spbte_launch_project_apply(&op->g, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
   else if (cons) LCHK(spbte_launch_project(&op->g, 1, op->masses, op->LU, op->pivot, d_Q, ncell, ss->part, op->s));
@@ -944,10 +958,27 @@ int spbte_compute_q(spbte_op *op, const double *d_f, const double *d_g, double *
     const size_t off = (size_t)c0 * op->n3;
     const scratch_set ss = own_scratch(op);
     rc = compute_chunk(op, d_f + off, d_g ? d_g + off : NULL, d_Q + off, nc, form, flags | (d_g ? 0 : SPBTE_SELF), NULL,
-                       NULL, &ss);
+                       NULL, &ss, NULL);
     if (rc) return rc;
   }
   return SPBTE_OK;
+}
+
+int spbte_compute_q_update(spbte_op *op, const double *d_f, double *d_Q, int ncell, int form, int flags,
+                           const spbte_epilogue *epi, int *fused) {
+  if (!op || !epi || !fused || ncell < 1 || form < 0 || form >= NFORMS || !d_f || !d_Q)
+    return fail(SPBTE_ERR_ARG, "bad arguments");
+  *fused = 0;
+  if (!op->t->have_w) return fail(SPBTE_ERR_NOWEIGHTS, "no weight table set");
+  if (ncell > chunk_cells(op) || !op->g.d_fft || !spbte_fft_supported(op->N) || !spbte_fft_cube(op->N, ncell) ||
+      ((flags & SPBTE_CONSERVE) && op->Ns != 1))
+    return spbte_compute_q(op, d_f, NULL, d_Q, ncell, form, flags);
+  int rc = ensure_scratch(op, ncell);
+  if (rc) return rc;
+  const scratch_set ss = own_scratch(op);
+  rc = compute_chunk(op, d_f, NULL, d_Q, ncell, form, flags | SPBTE_SELF, NULL, NULL, &ss, epi);
+  if (!rc) *fused = 1;
+  return rc;
 }
 
 /* Bytes rounded up to the workspace's 256-byte alignment. */
@@ -1000,7 +1031,7 @@ int spbte_compute_q_noalloc(spbte_op *op, const double *d_f, const double *d_g, 
   ss.Qp = ss.xs;
   ss.qp_bytes = b_x;
   ss.fixed = 1;
-  return compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags, NULL, NULL, &ss);
+  return compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags, NULL, NULL, &ss, NULL);
 }
 
 /* The operator's partial-sum scratch when it holds the split projection's
@@ -1109,7 +1140,7 @@ int spbte_time_compute_q(spbte_op *op, const double *d_f, const double *d_g, dou
   const scratch_set ss = own_scratch(op);
   for (int i = 0; i < iters && !rc; i++)
     rc = compute_chunk(op, d_f, d_g, d_Q, ncell, form, flags | (d_g ? 0 : SPBTE_SELF), ev[2 + 2 * i], ev[3 + 2 * i],
-                       &ss);
+                       &ss, NULL);
   if (!rc && (hipEventRecord(ev[1], op->s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
               hipEventElapsedTime(&tot, ev[0], ev[1]) != hipSuccess))
     rc = fail(SPBTE_ERR_HIP, "timing failed: %s", hipGetErrorString(hipGetLastError()));

@@ -40,6 +40,9 @@ typedef struct {
   void *dst;         /* complex output, double2 or float2 per element */
   const double *part; /* instead of mom (FFT grids): the per-plane moment partial
                          sums of spbte_launch_moment_partials, 5 N per cell */
+  const double *msrc; /* instead of mom and part (grids where
+                         spbte_fft_moments_inline holds): the cells whose
+                         moments the transform takes itself */
 } spbte_xform_job;
 
 /* Weighted-sum kernel arguments.  F/G arrays hold ncell * N^3 complex values:
@@ -82,6 +85,30 @@ int spbte_launch_xform_fwd_complex(const spbte_grid_dev *g, const void *in, void
  * may then alias scratch).  proj_part (FFT grids only, else NULL): also writes
  * the one-species projection's per-plane partial sums of q (5 N per cell) for
  * spbte_launch_project_apply. */
+/* 1 if forward transform jobs on an N grid may give msrc for their moments. */
+int spbte_fft_moments_inline(int N);
+/* 1 if spbte_launch_xform_inv_real_cube serves ncubes cells of an N grid
+ * (one workgroup per cube for the last pass). */
+int spbte_fft_cube(int N, int ncubes);
+/* An elementwise update fused into an evaluation's last pass: out = a x +
+ * b y + c Q over the evaluated cells (y may be NULL; out may be x). */
+typedef struct {
+  double a, b, c;
+  const double *x, *y;
+  double *out;
+} spbte_epilogue;
+/* spbte_compute_q (f alone, as g) with the update epi fused into its last pass
+ * where the grid and batch allow it (*fused = 1); otherwise the plain
+ * evaluation, and the caller applies the update (*fused = 0). */
+struct spbte_op;
+int spbte_compute_q_update(struct spbte_op *op, const double *d_f, double *d_Q, int ncell, int form, int flags,
+                           const spbte_epilogue *epi, int *fused);
+/* Inverse transform of the sum of nsl sets to the real part on whole-cube
+ * grids, projected per cell (one species) when masses is not NULL, then the
+ * update epi when it is not NULL. */
+int spbte_launch_xform_inv_real_cube(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell,
+                                     void *scratch, const double *masses, const double *LU, const int *pivot,
+                                     const spbte_epilogue *epi, hipStream_t s);
 int spbte_launch_xform_inv_real(const spbte_grid_dev *g, const void *Qh, int nsl, double *q, int ncell, void *scratch,
                                 double *proj_part, hipStream_t s);
 /* The first pass of spbte_launch_moments alone (part: 5 N per cell), whose

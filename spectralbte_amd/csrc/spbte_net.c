@@ -336,7 +336,7 @@ static int backward(spbte_net *n, int B) {
 static int moments_and_features(spbte_net *n, const double *d_f, int B) {
   const spbte_grid_dev *g = spbte_op_grid(n->op);
   LCHK(spbte_launch_moments(g, d_f, n->mom, B, NULL, n->s));
-  spbte_xform_job jobs[2] = {{d_f, n->mom, 1, n->Fd, NULL}, {d_f, n->mom, 2, n->Fm, NULL}};
+  spbte_xform_job jobs[2] = {{d_f, n->mom, 1, n->Fd, NULL, NULL}, {d_f, n->mom, 2, n->Fm, NULL, NULL}};
   LCHK(spbte_launch_xform_fwd(g, jobs, 2, B, 0, n->xs, n->s));
   LCHK(spbte_net_features(n->N, n->Kb, B, n->Fd, n->Fm, n->mom, n->fscale, n->X, n->s));
   return SPBTE_OK;

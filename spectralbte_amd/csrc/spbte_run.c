@@ -387,6 +387,20 @@ static int update(spbte_run *r, double a, const double *x, size_t xs, double b, 
   return SPBTE_OK;
 }
 
+/* collide(d_in), then update(...); with one species and the exact operator the
+ * update rides on the evaluation's last pass where the operator can fuse it. */
+static int collide_update(spbte_run *r, const double *d_in, size_t stride, double a, const double *x, size_t xs,
+                          double b, const double *y, size_t ys, double c, double *out, size_t os) {
+  if (r->ns == 1 && !r->net) {
+    const spbte_epilogue ep = {a, b, c, x, y, out};
+    int fused = 0;
+    CHK(spbte_compute_q_update(r->op, d_in, r->d_Q, r->ncell, r->form, r->flags, &ep, &fused));
+    return fused ? SPBTE_OK : update(r, a, x, xs, b, y, ys, c, out, os);
+  }
+  CHK(collide(r, d_in, stride));
+  return update(r, a, x, xs, b, y, ys, c, out, os);
+}
+
 static int advect_all(spbte_run *r, double *src, double *dst) {
   for (int l = 0; l < r->ns; l++) CHK(advect(r, src + l * gblock(r), dst + l * gblock(r), l));
   return SPBTE_OK;
@@ -398,24 +412,16 @@ static int enqueue_step(spbte_run *r) {
   double *f = interior(r, r->d_f);
   if (r->dim == 0) {
     /* exec/boltz.c:193-241 */
-    CHK(collide(r, f, G));
-    if (r->order == 1) {
-      CHK(update(r, 1.0, f, G, 0.0, NULL, 0, h, f, G));
-    } else {
-      CHK(update(r, 1.0, f, G, 0.0, NULL, 0, h, r->d_f1, I));
-      CHK(collide(r, r->d_f1, I));
-      CHK(update(r, 0.5, f, G, 0.5, r->d_f1, I, 0.5 * h, f, G));
-    }
-    return SPBTE_OK;
+    if (r->order == 1) return collide_update(r, f, G, 1.0, f, G, 0.0, NULL, 0, h, f, G);
+    CHK(collide_update(r, f, G, 1.0, f, G, 0.0, NULL, 0, h, r->d_f1, I));
+    return collide_update(r, r->d_f1, I, 0.5, f, G, 0.5, r->d_f1, I, 0.5 * h, f, G);
   }
   /* exec/boltz.c:285-354 */
   double *fc = interior(r, r->d_fc);
   CHK(advect_all(r, r->d_f, r->d_fc));
-  CHK(collide(r, fc, G));
-  if (r->order == 1) return update(r, 1.0, fc, G, 0.0, NULL, 0, h, f, G);
-  CHK(update(r, 1.0, fc, G, 0.0, NULL, 0, h, r->d_f1, I));
-  CHK(collide(r, r->d_f1, I));
-  CHK(update(r, 0.5, fc, G, 0.5, r->d_f1, I, 0.5 * h, fc, G));
+  if (r->order == 1) return collide_update(r, fc, G, 1.0, fc, G, 0.0, NULL, 0, h, f, G);
+  CHK(collide_update(r, fc, G, 1.0, fc, G, 0.0, NULL, 0, h, r->d_f1, I));
+  CHK(collide_update(r, r->d_f1, I, 0.5, fc, G, 0.5, r->d_f1, I, 0.5 * h, fc, G));
   return advect_all(r, r->d_fc, r->d_f);
 }
 
