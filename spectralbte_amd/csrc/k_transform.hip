@@ -794,6 +794,17 @@ __global__ void __launch_bounds__(XT) k_fft_cube_inv_x(int ncell, const double2 
   double2 x[N];
 #pragma unroll
   for (int a = 0; a < N; ++a) x[a] = own ? scratch[cell * n3 + (size_t)a * n2 + tid] : make_double2(0.0, 0.0);
+  // the update's a x + b y, loaded with the transform's input (every load
+  // before the first store: out may alias x)
+  double r[N];
+  if (ep.out && own) {
+#pragma unroll
+    for (int a = 0; a < N; ++a) r[a] = ep.a * ep.x[cell * n3 + (size_t)a * n2 + tid];
+    if (ep.y) {
+#pragma unroll
+      for (int a = 0; a < N; ++a) r[a] += ep.b * ep.y[cell * n3 + (size_t)a * n2 + tid];
+    }
+  }
   __syncthreads();
   double qv[N];
   dft_reg<N>(x, t.tw, dsgn);
@@ -828,14 +839,7 @@ __global__ void __launch_bounds__(XT) k_fft_cube_inv_x(int ncell, const double2 
   if (own) {
 #pragma unroll
     for (int a = 0; a < N; ++a) q[cell * n3 + (size_t)a * n2 + tid] = qv[a];
-    if (ep.out) {  // every load before the first store: out may alias x
-      double r[N];
-#pragma unroll
-      for (int a = 0; a < N; ++a) r[a] = ep.a * ep.x[cell * n3 + (size_t)a * n2 + tid];
-      if (ep.y) {
-#pragma unroll
-        for (int a = 0; a < N; ++a) r[a] += ep.b * ep.y[cell * n3 + (size_t)a * n2 + tid];
-      }
+    if (ep.out) {
 #pragma unroll
       for (int a = 0; a < N; ++a) ep.out[cell * n3 + (size_t)a * n2 + tid] = r[a] + ep.c * qv[a];
     }

@@ -145,3 +145,46 @@ def test_abi_table_cache_holds_a_three_species_step(spb, orc, capfd):
             assert rel_l2(spb.ComputeQ(f, g, R), orc.compute_q(N, v, eta, W, f, g)) < 1e-12
     assert "evicting" not in capfd.readouterr().err
     spb.dealloc_coll()
+
+
+@pytest.mark.parametrize("masses", [(1.0,), (1.0, 2.0, 0.5)])
+def test_projection_kernels_against_oracle(spb, orc, masses):
+    """The two projection paths of spbte_conserve_species against the oracle's
+    conserveAllMoments (src/conserve.c:207-264): one workgroup per group
+    (k_project, an operator with no partial-sum scratch yet) and the split
+    (x-plane, group) kernels k_proj_part/k_proj_apply (after an evaluation
+    has sized the scratch), for one and three species; each <= 1e-12 relative
+    to the oracle and the projected arrays conserve to 1e-12."""
+    from conftest import rel_l2
+    N, Lv, ngroups = 8, 5.0, 3
+    Ns = len(masses)
+    v, eta = orc.grid_0d(N, Lv)
+    n3 = N ** 3
+    rng = np.random.default_rng(0x5EED0007)
+    Q = rng.standard_normal((ngroups, Ns * Ns, n3)) * 1e-2
+    cons = orc.Conserver(N, v, masses)
+    ref = np.stack([cons(Q[g]) for g in range(ngroups)])
+    op = spb.Operator(N, Lv, v, eta)
+    op.set_species(masses)
+    one_wg = op.conserve_species(Q, Ns)
+    op.generate_weights_iso(1.0, 2.0, 2.0, masses[0], masses[0])
+    f = np.abs(rng.standard_normal((4, n3)))
+    op.compute_q(f)  # sizes the operator's scratch for 4 cells
+    split = op.conserve_species(Q, Ns)
+    op.close()
+    w = np.ones(N); w[0] = w[-1] = 0.5
+    W3 = ((w[:, None, None] * w[None, :, None] * w[None, None, :]) * (v[1] - v[0]) ** 3).ravel()
+    V = [np.repeat(v, N * N), np.tile(np.repeat(v, N), N), np.tile(v, N * N)]
+    E = 0.5 * sum(Vd ** 2 for Vd in V)
+    for got in (one_wg, split):
+        for g in range(ngroups):
+            assert rel_l2(got[g], ref[g]) < 1e-12
+            # each species' mass, then the total momentum and energy, relative
+            # to the same sums of |terms|
+            qs = [sum(got[g][n * Ns + m] for n in range(Ns)) for m in range(Ns)]
+            for m in range(Ns):
+                assert abs(W3 @ qs[m]) <= 1e-12 * (W3 @ np.abs(qs[m]))
+            for row in V + [E]:
+                tot = sum(masses[m] * (W3 * row) @ qs[m] for m in range(Ns))
+                mag = sum(masses[m] * (W3 * np.abs(row)) @ np.abs(qs[m]) for m in range(Ns))
+                assert abs(tot) <= 1e-12 * mag
