@@ -958,8 +958,9 @@ extern "C" int spbte_launch_xform_fwd(const spbte_grid_dev *g, const spbte_xform
   XformJobs J = {};
   for (int q = 0; q < njobs; ++q) J.j[q] = jobs[q];
   for (int q = 0; q < njobs; ++q)
-    if (jobs[q].mode != 0 && !jobs[q].mom && !jobs[q].part && (!jobs[q].msrc || !spbte_fft_moments_inline(g->N)))
-      return (int)hipErrorInvalidValue;
+    if (jobs[q].mode != 0 && !jobs[q].mom && !jobs[q].part &&
+        (!jobs[q].msrc || !g->d_fft || !spbte_fft_moments_inline(g->N)))
+      return (int)hipErrorInvalidValue;  // msrc: FFT grids with N <= 16 only
   if (g->d_fft && with_fft_n(g->N, [&](auto n) {
         launch_fft_fwd<decltype(n)::value>(g, J, njobs, ncell, out_fp32, scratch, s);
       }))
