@@ -34,6 +34,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+# The shock's domain length: 256 cells of 0.039, so the order-2 stages' Courant
+# number 0.5 dt |v| / dx stays at 0.21 or less for |v| <= L_v = 8 (on a unit
+# domain it is 2.0 and the run diverges within 50 steps).
+SHOCK_LENGTH = 10.0
+# The heat-transfer run's time step with the unnormalised glancing-collision
+# table (1e-6 and 3e-7 diverge within 200 steps).
+HEAT_DT = 1e-7
 FP64_VALU_TFLOPS = 78.6    # MI355X FP64 vector peak (FP64-arithmetic forms)
 FP32_VALU_TFLOPS = 157.3   # MI355X FP32 vector peak as published (it counts packed FP32, which this build does not issue)
 
@@ -66,6 +73,8 @@ def parse():
     p.add_argument("--no-extra", action="store_true", help="skip every row beside the headline")
     p.add_argument("--no-runs", action="store_true", help="skip the whole-step runs (shock, mixed ensemble)")
     p.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
+    p.add_argument("--full-runs", action="store_true",
+                   help="also run configurations 2-4 at their full lengths (about 4 minutes)")
     p.add_argument("--rows", default=None,
                    help="comma-separated secondary rows to run, e.g. c1_bkw16 (default: every row the flags above keep)")
     return p.parse_args()
@@ -245,6 +254,8 @@ def main():
         row("heat_glance_run", "heat-transfer run", heat_glance_run)
     if not a.no_extra and not a.no_surrogate:
         row("surrogate", "surrogate", surrogate_run)
+    if a.full_runs:
+        row("full_runs", "full-length runs", full_runs)
 
     cpu = None
     if not a.no_cpu:
@@ -343,8 +354,9 @@ def c3_shock_forms(spb, ncell=256):
 
 
 def shock_run(spb, steps=3):
-    """Whole steps of the 1D-3V Mach-1.2 shock (256 cells, N=32, L_v=8, lambda=1,
-    Kn=0.1, order 2, dt=0.002): advectTwo, Heun collision step (2 Q per cell),
+    """Whole steps of the 1D-3V Mach-1.2 shock (256 cells on a domain of length
+    SHOCK_LENGTH, N=32, L_v=8, lambda=1, Kn=0.1, order 2, dt=0.002):
+    advectTwo, Heun collision step (2 Q per cell),
     advectTwo, replayed as one captured HIP graph per step; in FP64 (the form
     that meets 1e-6 on ComputeQ near equilibrium) and in the FP32-product form,
     with that run's state change against the FP64 run's."""
@@ -353,7 +365,7 @@ def shock_run(spb, steps=3):
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
     op.generate_weights_iso(1.0)
-    x, dx = _mesh(nX, 1.0, 2)
+    x, dx = _mesh(nX, SHOCK_LENGTH, 2)
     f0 = workloads.shock_profile(N, Lv, nX)
     res, states = {}, {}
     for form in ("fp64", "fp32a"):
@@ -373,8 +385,8 @@ def shock_run(spb, steps=3):
                      "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": bool(np.all(np.isfinite(states[form])))}
     res["fp32a"]["state_change_rel_err_vs_fp64"] = _state_change_error(f0, states["fp32a"], states["fp64"])
     op.close()
-    return {"workload": "1D-3V Mach-1.2 shock, 256 cells, N=32, L_v=8, lambda=1, Kn=0.1, order 2 (advectTwo + "
-                        "Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
+    return {"workload": f"1D-3V Mach-1.2 shock, 256 cells on length {SHOCK_LENGTH:g}, N=32, L_v=8, lambda=1, Kn=0.1, "
+                        "order 2 (advectTwo + Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
             "steps_timed": steps, **res}
 
 
@@ -424,8 +436,9 @@ def heat_glance_run(spb, steps=3):
     the FP32-product form, with the latter's error against FP64 on the
     collision operator of the initial state and on the runs' state change.
     The table carries no trapezoid or h_eta^3 factors, as the reference writes
-    it, so Q is about 1e4 times the isotropic table's and the explicit step is
-    stable near dt=1e-6 (2000 steps cover 2e-3 time units)."""
+    it, so Q is about 1e4 times the isotropic table's and the explicit step
+    needs dt = HEAT_DT = 1e-7 (1e-6 and 3e-7 diverge within 200 steps; 2000
+    steps cover 2e-4 time units)."""
     from spectralbte_amd import workloads
     N, Lv, nX = 32, 7.0, 128
     v, eta = workloads.grid_1d(N, Lv)
@@ -439,7 +452,7 @@ def heat_glance_run(spb, steps=3):
     for form in ("fp64", "fp32a"):
         tile = op.autotune(nX, form, False)
         qs[form] = op.compute_q(f0, form=form, conserve=True)
-        run = spb.Run.mesh(op, f0, x, dx, order=1, dt=1e-6, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
+        run = spb.Run.mesh(op, f0, x, dx, order=1, dt=HEAT_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
         run.use_graph(True)
         run.step(1)
         run.set_state(f0)
@@ -459,8 +472,111 @@ def heat_glance_run(spb, steps=3):
     res["fp32a"]["meets_1e-6"] = bool(e_q <= 1e-6)
     op.close()
     return {"workload": "1D-3V heat transfer, 128 cells, N=32, L_v=7, lambda=1, glancing-collision table "
-                        "(glance 0.1), Kn=0.05, order 1, diffuse walls T=1 and T=2, dt=1e-6, whole step as one HIP graph",
+                        f"(glance 0.1), Kn=0.05, order 1, diffuse walls T=1 and T=2, dt={HEAT_DT:g}, whole step as one "
+                        "HIP graph",
             "steps_timed": steps, **res}
+
+
+def full_runs(spb):
+    """Configurations 2-4 at their full lengths, each as one timed sequence of
+    graph-replayed steps: the mixed ensemble's 50 Heun steps (FP32 products and
+    FP64, conservation of every problem and the FP32 run's state change
+    against FP64), the shock's 500 steps in FP64 (the far-field states, which
+    the Rankine-Hugoniot jump keeps, and the profile's range), and the heat
+    transfer's 2000 steps in FP64 (mass between the diffuse walls)."""
+    from spectralbte_amd import workloads
+    out = {}
+    # C2: 512 problems at N=24 and 512 at N=32, L_v=6, lambda=1, Heun dt=0.005, 50 steps
+    c2 = {}
+    for N, seed in ((24, 0x5EED0024), (32, 0x5EED0032)):
+        v, eta = workloads.grid_0d(N, 6.0)
+        op = spb.Operator(N, 6.0, v, eta)
+        op.generate_weights_iso(1.0)
+        f0 = workloads.two_maxwellian_ensemble(N, 6.0, 512, seed=seed)
+        C = cmat(N, v)
+        ends = {}
+        for form in ("fp32a", "fp64"):
+            op.autotune(512, form, True)
+            r = spb.Run.ensemble(op, f0, order=2, dt=0.005, Kn=1.0, form=form)
+            r.use_graph(True)
+            r.step(1)  # captures the step's graph
+            r.set_state(f0)
+            op.sync()
+            t0 = time.perf_counter()
+            r.step(50)
+            sec = time.perf_counter() - t0
+            ends[form] = r.state()
+            r.close()
+            drift = np.abs(ends[form] @ C.T - f0 @ C.T) / (np.abs(f0) @ np.abs(C).T)
+            c2[f"N{N}_{form}"] = {"seconds": round(sec, 2), "q_per_s": round(2 * 512 * 50 / sec, 1),
+                                  "conserved_moment_drift_max_rel": float(drift.max()),
+                                  "finite": bool(np.all(np.isfinite(ends[form])))}
+            log(f"full C2 N={N} {form}: 50 steps in {sec:.2f} s, drift {drift.max():.2e}")
+        c2[f"N{N}_fp32a"]["state_change_rel_err_vs_fp64"] = _state_change_error(f0, ends["fp32a"], ends["fp64"])
+        op.close()
+    out["c2_mixed_ensemble_50_steps"] = c2
+    # C3: 256-cell Mach-1.2 shock, N=32, L_v=8, Kn=0.1, order 2, dt=0.002, 500 steps, FP64
+    N, Lv, nX = 32, 8.0, 256
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    x, dx = _mesh(nX, SHOCK_LENGTH, 2)
+    f0 = workloads.shock_profile(N, Lv, nX)
+    op.autotune(nX, "fp64", False)
+    r = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form="fp64")
+    r.use_graph(True)
+    r.step(1)
+    r.set_state(f0)
+    m0 = r.moments()
+    op.sync()
+    t0 = time.perf_counter()
+    r.step(500)
+    sec = time.perf_counter() - t0
+    m1 = r.moments()
+    fin = bool(np.all(np.isfinite(r.state())))
+    r.close()
+    op.close()
+    far = {"upstream_start": [round(float(q), 6) for q in m0[0][[0, 1, 4]]],
+           "upstream_end": [round(float(q), 6) for q in m1[0][[0, 1, 4]]],
+           "downstream_start": [round(float(q), 6) for q in m0[-1][[0, 1, 4]]],
+           "downstream_end": [round(float(q), 6) for q in m1[-1][[0, 1, 4]]]}
+    out["c3_shock_500_steps_fp64"] = {"seconds": round(sec, 2), "q_per_s": round(2 * nX * 500 / sec, 1),
+                                      "ms_per_step": round(sec * 2.0, 3), "finite": fin,
+                                      "rho_range_end": [float(m1[:, 0].min()), float(m1[:, 0].max())],
+                                      "T_range_end": [float(m1[:, 4].min()), float(m1[:, 4].max())],
+                                      "far_field_rho_ux_T": far}
+    log(f"full C3: 500 steps in {sec:.1f} s; far field {far}")
+    # C4: heat transfer between diffuse walls, glance 0.1, 128 cells, dt=HEAT_DT, 2000 steps, FP64
+    N, Lv, nX = 32, 7.0, 128
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_glance(1.0, 0.1)
+    x, dx = _mesh(nX, 1.0, 1)
+    noise = 1.0 + 0.01 * np.random.default_rng(128).standard_normal(nX)
+    V2 = (v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2).ravel()
+    f0 = np.stack([rr * np.exp(-V2 / 1.5) / (1.5 * np.pi) ** 1.5 for rr in noise])
+    op.autotune(nX, "fp64", False)
+    r = spb.Run.mesh(op, f0, x, dx, order=1, dt=HEAT_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form="fp64")
+    r.use_graph(True)
+    r.step(1)
+    r.set_state(f0)
+    m0 = r.moments()
+    op.sync()
+    t0 = time.perf_counter()
+    r.step(2000)
+    sec = time.perf_counter() - t0
+    m1 = r.moments()
+    fin = bool(np.all(np.isfinite(r.state())))
+    r.close()
+    op.close()
+    d = dx[1:-1]
+    mass0, mass1 = float(m0[:, 0] @ d), float(m1[:, 0] @ d)
+    out["c4_heat_2000_steps_fp64"] = {"seconds": round(sec, 2), "q_per_s": round(nX * 2000 / sec, 1),
+                                      "ms_per_step": round(sec / 2.0, 3), "finite": fin,
+                                      "mass_rel_change": abs(mass1 - mass0) / mass0,
+                                      "T_wall_cells_end": [float(m1[0, 4]), float(m1[-1, 4])]}
+    log(f"full C4: 2000 steps in {sec:.1f} s, mass change {abs(mass1 - mass0) / mass0:.2e}")
+    return out
 
 
 def _mesh(nX, length, order):
