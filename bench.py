@@ -34,13 +34,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-# The shock's domain length: 256 cells of 0.039, so the order-2 stages' Courant
-# number 0.5 dt |v| / dx stays at 0.21 or less for |v| <= L_v = 8 (on a unit
-# domain it is 2.0 and the run diverges within 50 steps).
-SHOCK_LENGTH = 10.0
-# The heat-transfer run's time step with the unnormalised glancing-collision
-# table (1e-6 and 3e-7 diverge within 200 steps).
-HEAT_DT = 1e-7
 FP64_VALU_TFLOPS = 78.6    # MI355X FP64 vector peak (FP64-arithmetic forms)
 FP32_VALU_TFLOPS = 157.3   # MI355X FP32 vector peak as published (it counts packed FP32, which this build does not issue)
 
@@ -355,8 +348,8 @@ def c3_shock_forms(spb, ncell=256):
 
 def shock_run(spb, steps=3):
     """Whole steps of the 1D-3V Mach-1.2 shock (256 cells on a domain of length
-    SHOCK_LENGTH, N=32, L_v=8, lambda=1, Kn=0.1, order 2, dt=0.002):
-    advectTwo, Heun collision step (2 Q per cell),
+    workloads.SHOCK_LENGTH = 10, N=32, L_v=8, lambda=1, Kn=0.1, order 2,
+    dt=0.002): advectTwo, Heun collision step (2 Q per cell),
     advectTwo, replayed as one captured HIP graph per step; in FP64 (the form
     that meets 1e-6 on ComputeQ near equilibrium) and in the FP32-product form,
     with that run's state change against the FP64 run's."""
@@ -365,7 +358,7 @@ def shock_run(spb, steps=3):
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
     op.generate_weights_iso(1.0)
-    x, dx = _mesh(nX, SHOCK_LENGTH, 2)
+    x, dx = _mesh(nX, workloads.SHOCK_LENGTH, 2)
     f0 = workloads.shock_profile(N, Lv, nX)
     res, states = {}, {}
     for form in ("fp64", "fp32a"):
@@ -385,8 +378,9 @@ def shock_run(spb, steps=3):
                      "projected_500_steps_s": round(500 * ms / 1e3, 1), "finite": bool(np.all(np.isfinite(states[form])))}
     res["fp32a"]["state_change_rel_err_vs_fp64"] = _state_change_error(f0, states["fp32a"], states["fp64"])
     op.close()
-    return {"workload": f"1D-3V Mach-1.2 shock, 256 cells on length {SHOCK_LENGTH:g}, N=32, L_v=8, lambda=1, Kn=0.1, "
-                        "order 2 (advectTwo + Heun collision + advectTwo), dt=0.002, whole step as one HIP graph",
+    return {"workload": f"1D-3V Mach-1.2 shock, 256 cells on length {workloads.SHOCK_LENGTH:g}, N=32, L_v=8, "
+                        "lambda=1, Kn=0.1, order 2 (advectTwo + Heun collision + advectTwo), dt=0.002, whole step "
+                        "as one HIP graph",
             "steps_timed": steps, **res}
 
 
@@ -437,8 +431,8 @@ def heat_glance_run(spb, steps=3):
     collision operator of the initial state and on the runs' state change.
     The table carries no trapezoid or h_eta^3 factors, as the reference writes
     it, so Q is about 1e4 times the isotropic table's and the explicit step
-    needs dt = HEAT_DT = 1e-7 (1e-6 and 3e-7 diverge within 200 steps; 2000
-    steps cover 2e-4 time units)."""
+    needs dt = workloads.HEAT_GLANCE_DT = 1e-7 (1e-6 and 3e-7 diverge within
+    200 steps; 2000 steps cover 2e-4 time units)."""
     from spectralbte_amd import workloads
     N, Lv, nX = 32, 7.0, 128
     v, eta = workloads.grid_1d(N, Lv)
@@ -452,7 +446,7 @@ def heat_glance_run(spb, steps=3):
     for form in ("fp64", "fp32a"):
         tile = op.autotune(nX, form, False)
         qs[form] = op.compute_q(f0, form=form, conserve=True)
-        run = spb.Run.mesh(op, f0, x, dx, order=1, dt=HEAT_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
+        run = spb.Run.mesh(op, f0, x, dx, order=1, dt=workloads.HEAT_GLANCE_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form=form)
         run.use_graph(True)
         run.step(1)
         run.set_state(f0)
@@ -472,8 +466,8 @@ def heat_glance_run(spb, steps=3):
     res["fp32a"]["meets_1e-6"] = bool(e_q <= 1e-6)
     op.close()
     return {"workload": "1D-3V heat transfer, 128 cells, N=32, L_v=7, lambda=1, glancing-collision table "
-                        f"(glance 0.1), Kn=0.05, order 1, diffuse walls T=1 and T=2, dt={HEAT_DT:g}, whole step as one "
-                        "HIP graph",
+                        f"(glance 0.1), Kn=0.05, order 1, diffuse walls T=1 and T=2, dt={workloads.HEAT_GLANCE_DT:g}, "
+                        "whole step as one HIP graph",
             "steps_timed": steps, **res}
 
 
@@ -520,7 +514,7 @@ def full_runs(spb):
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
     op.generate_weights_iso(1.0)
-    x, dx = _mesh(nX, SHOCK_LENGTH, 2)
+    x, dx = _mesh(nX, workloads.SHOCK_LENGTH, 2)
     f0 = workloads.shock_profile(N, Lv, nX)
     op.autotune(nX, "fp64", False)
     r = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form="fp64")
@@ -546,7 +540,7 @@ def full_runs(spb):
                                       "T_range_end": [float(m1[:, 4].min()), float(m1[:, 4].max())],
                                       "far_field_rho_ux_T": far}
     log(f"full C3: 500 steps in {sec:.1f} s; far field {far}")
-    # C4: heat transfer between diffuse walls, glance 0.1, 128 cells, dt=HEAT_DT, 2000 steps, FP64
+    # C4: heat transfer between diffuse walls, glance 0.1, 128 cells, dt=1e-7, 2000 steps, FP64
     N, Lv, nX = 32, 7.0, 128
     v, eta = workloads.grid_1d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
@@ -556,7 +550,7 @@ def full_runs(spb):
     V2 = (v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2).ravel()
     f0 = np.stack([rr * np.exp(-V2 / 1.5) / (1.5 * np.pi) ** 1.5 for rr in noise])
     op.autotune(nX, "fp64", False)
-    r = spb.Run.mesh(op, f0, x, dx, order=1, dt=HEAT_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form="fp64")
+    r = spb.Run.mesh(op, f0, x, dx, order=1, dt=workloads.HEAT_GLANCE_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form="fp64")
     r.use_graph(True)
     r.step(1)
     r.set_state(f0)

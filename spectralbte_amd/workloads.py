@@ -11,6 +11,17 @@ import numpy as np
 
 GAMMA = 5.0 / 3.0  # monatomic gas
 
+# The Mach-1.2 shock's domain length: its 256 cells are then 0.039 wide, so
+# the order-2 stages' Courant number 0.5 dt |v| / dx stays at 0.21 or less for
+# |v| <= L_v = 8 at dt = 0.002 (on a unit domain it is 2.0 and the run
+# diverges within 50 steps).
+SHOCK_LENGTH = 10.0
+# The heat-transfer run's time step with the glancing-collision table as the
+# reference writes it (no trapezoid or h_eta^3 factors, so Q is about 1e4
+# times the isotropic table's): at Kn = 0.05 the explicit step diverges
+# within 200 steps at 1e-6 and 3e-7.
+HEAT_GLANCE_DT = 1e-7
+
 
 def grid_0d(N, Lv):
     """src/initializer.c:66-82."""

@@ -259,3 +259,60 @@ def test_c1_bkw16_run_against_oracle(spb, orc):
     assert abs(rows[0][1] - 1.0) < 1e-6 and abs(rows[0][3] - 1.0) < 1e-5 and abs(rows[0][2]) < 1e-15
     run.close()
     op.close()
+
+
+def _mesh(nX, length, order):
+    d = length / nX
+    return (np.arange(nX + 2 * order) - order + 0.5) * d, np.full(nX + 2 * order, d)
+
+
+def test_shock_and_heat_runs_stay_bounded(spb):
+    """The 1D runs at their configured sizes with the parameters that keep the
+    explicit steps stable (exec/boltz.c:285-354): the shock on its domain of
+    length workloads.SHOCK_LENGTH for 100 order-2 steps (dt=0.002, Kn=0.1),
+    and the heat transfer on the glancing-collision table at dt =
+    workloads.HEAT_GLANCE_DT for 300 order-1 steps (Kn=0.05).  Bounds: every
+    value finite; the shock's far-field cells keep their Rankine-Hugoniot
+    states to 1e-6 and its density stays within [0.999, 1.35]; the heat
+    transfer's cell densities move by less than 0.02 and its mass between the
+    diffuse walls is kept to 1e-6.  (On a unit domain the shock's Courant number is 2.0 and, at
+    dt=1e-6, the heat transfer's collision step is unstable: both diverge
+    within 200 steps.)"""
+    from spectralbte_amd import workloads
+    N, Lv, nX = 32, 8.0, 256
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_iso(1.0)
+    x, dx = _mesh(nX, workloads.SHOCK_LENGTH, 2)
+    f0 = workloads.shock_profile(N, Lv, nX)
+    r = spb.Run.mesh(op, f0, x, dx, order=2, dt=0.002, Kn=0.1, bc=spb.BC_COPY, form="fp64")
+    r.use_graph(True)
+    m0 = r.moments()
+    r.step(100)
+    m1 = r.moments()
+    assert np.all(np.isfinite(r.state()))
+    r.close()
+    op.close()
+    for c in (0, -1):
+        assert np.abs(m1[c] - m0[c]).max() <= 1e-6
+    assert 0.999 <= m1[:, 0].min() and m1[:, 0].max() <= 1.35
+
+    N, Lv, nX = 32, 7.0, 128
+    v, eta = workloads.grid_1d(N, Lv)
+    op = spb.Operator(N, Lv, v, eta)
+    op.generate_weights_glance(1.0, 0.1)
+    x, dx = _mesh(nX, 1.0, 1)
+    noise = 1.0 + 0.01 * np.random.default_rng(128).standard_normal(nX)
+    V2 = (v[:, None, None] ** 2 + v[None, :, None] ** 2 + v[None, None, :] ** 2).ravel()
+    f0 = np.stack([s * np.exp(-V2 / 1.5) / (1.5 * np.pi) ** 1.5 for s in noise])
+    r = spb.Run.mesh(op, f0, x, dx, order=1, dt=workloads.HEAT_GLANCE_DT, Kn=0.05, bc=spb.BC_DIFFUSE, form="fp64")
+    r.use_graph(True)
+    m0 = r.moments()
+    r.step(300)
+    m1 = r.moments()
+    assert np.all(np.isfinite(r.state()))
+    r.close()
+    op.close()
+    assert np.abs(m1[:, 0] - m0[:, 0]).max() <= 0.02  # the 1% noise relaxes, no cell runs away
+    mass0, mass1 = m0[:, 0] @ dx[1:-1], m1[:, 0] @ dx[1:-1]
+    assert abs(mass1 - mass0) <= 1e-6 * mass0
