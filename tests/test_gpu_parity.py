@@ -342,10 +342,14 @@ def test_heat_run_batched(spb, orc):
 
 
 # ------------------------------------------------------------ batched path
-@pytest.mark.parametrize("N,Lv,lam,ncell", [(16, 5.0, 0.0, 5), (24, 6.0, 1.0, 3)])
+@pytest.mark.parametrize("N,Lv,lam,ncell", [(16, 5.0, 0.0, 5), (24, 6.0, 1.0, 3), (6, 4.0, 1.0, 2), (9, 5.0, 1.0, 2),
+                                          (12, 5.0, 0.0, 3)])
 def test_batched_compute_q_fp64(spb, orc, N, Lv, lam, ncell):
     """Batched ComputeQ / maxPreserve (+ projection) on physical inputs vs the oracle,
-    W generated on the GPU and checked against the oracle's table."""
+    W generated on the GPU and checked against the oracle's table.  Up to N=16
+    the forward transform sums the Maxwellians' moments itself and the inverse
+    transform's last pass runs over whole cubes with the projection (N = 6, 9,
+    12, 16: radix 2, 3 and 4 stages in registers)."""
     v, eta = orc.grid_0d(N, Lv)
     op = spb.Operator(N, Lv, v, eta)
     op.generate_weights_iso(lam)
