@@ -883,10 +883,12 @@ static int compute_chunk(spbte_op *op, const double *d_f, const double *d_g, dou
    * from per-plane partial sums (f's, then g's, in ss->part), and the inverse
    * transform writes the projection's partial sums (over f's, dead by then) */
   const int fused = op->g.d_fft != NULL && spbte_fft_supported(op->N);
-  /* small grids: the forward transform's workgroups take the Maxwellians'
-   * moments from the cells themselves, and the inverse transform's last pass
-   * runs over whole cubes and projects in the same launch (few cells) */
-  const int mom_inl = fused && spbte_fft_moments_inline(op->N);
+  /* small grids: for a few cells the forward transform's workgroups take the
+   * Maxwellians' moments from the cells themselves (every Maxwellian
+   * workgroup reads its whole cell, 1 MB per cell at N=16, against a launch
+   * of the partial-sum pass), and the inverse transform's last pass runs over
+   * whole cubes and projects in the same launch */
+  const int mom_inl = fused && spbte_fft_moments_inline(op->N) && ncell <= 32;
   const int cube_i = fused && spbte_fft_cube(op->N, ncell);
   const double *part_f = fused && !mom_inl ? ss->part : NULL;
   const double *part_g = fused && !mom_inl ? ss->part + (size_t)5 * op->N * ncell : NULL;
